@@ -1,0 +1,184 @@
+/*
+ * sbwt_gpu_amd — SBWT k-mer membership search on one AMD Instinct MI355X.
+ *
+ * C ABI of libsbwt_gpu_amd.so. Plain pointers and sizes only. Every function
+ * returns SBWT_OK (0) or an error code; sbwt_last_error() gives the message
+ * of the calling thread's last failure. Pointers named d_* are device (HBM)
+ * pointers; `stream` is a hipStream_t (NULL = the null stream). Calls on one
+ * index are thread-compatible, not thread-safe: do not mutate an index
+ * (presearch, relayout, free) while another thread searches it.
+ *
+ * Bit order of rows: bit i of a row is bit 63-(i&63) of word i>>6
+ * (MSB-first), the order the reference reaches after its byte swap
+ * (include/rank_data_structures.hpp:98). Node ids are SBWT column indices; an
+ * absent k-mer answers ~0 (the reference's -1, include/gpu/search_kernel.hpp:98-103).
+ *
+ * Reference interfaces each entry point replaces are cited per function
+ * (paths relative to the epicharri/sbwt_gpu tree).
+ */
+#ifndef SBWT_GPU_AMD_H
+#define SBWT_GPU_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference returns int 0/1, include/gpu/cuda.hpp:7-11) */
+#define SBWT_OK 0
+#define SBWT_ERR_ARG 1         /* invalid argument or size */
+#define SBWT_ERR_HIP 2         /* HIP runtime error */
+#define SBWT_ERR_IO 3          /* file missing / short */
+#define SBWT_ERR_FORMAT 4      /* inconsistent index (unequal rows, C[4] != n) */
+#define SBWT_ERR_UNSUPPORTED 5 /* configuration this build does not support */
+
+/* ---- rank layouts: bytes per rank chunk (u32 count + payload bits) */
+#define SBWT_LAYOUT_AUTO 0
+#define SBWT_LAYOUT_L16 16 /* 96 columns per 16-byte chunk  */
+#define SBWT_LAYOUT_L32 32 /* 224 columns per 32-byte chunk */
+#define SBWT_LAYOUT_L64 64 /* 480 columns per 64-byte chunk */
+
+/* ---- flags of the read search */
+#define SBWT_READS_N_AS_A 1u      /* code non-ACGT as A and search it (reference
+                                     packing table, create_kmer_string.hpp:20);
+                                     default: a window with a non-ACGT char answers ~0 */
+#define SBWT_READS_INDEPENDENT 2u /* search every k-mer from the presearch table
+                                     (the reference's way); default: stream
+                                     consecutive k-mers through suffix groups */
+
+typedef struct sbwt_index sbwt_index;
+
+typedef struct sbwt_config {
+  uint32_t presearch_depth; /* m; 0 = no presearch table */
+  uint32_t layout;          /* SBWT_LAYOUT_* */
+  uint32_t block_threads;   /* threads per block of the search kernels, 0 = auto */
+  uint32_t segment_kmers;   /* k-mers per work item of the read search, 0 = auto */
+  uint32_t variant;         /* search kernel variant, 0 = auto */
+  uint32_t reserved[3];
+} sbwt_config;
+
+typedef struct sbwt_index_info {
+  uint64_t n;               /* columns (nodes) */
+  uint64_t C[5];            /* C[0]=1, C[c+1]=C[c]+ones(c) */
+  uint64_t device_bytes;    /* HBM held by the index */
+  uint32_t k;
+  uint32_t presearch_depth;
+  uint32_t layout;
+  uint32_t has_groups;      /* suffix-group starts present (streaming search) */
+} sbwt_index_info;
+
+/* Defaults: presearch_depth 12 (include/parameters.hpp:50), layout L16. */
+int sbwt_config_init(sbwt_config *cfg);
+
+/* Index from four host rows (A,C,G,T; each >= ceil(n/64) words) plus the
+ * optional suffix-group-start row (NULL: streaming is unavailable).
+ * Replaces epic::MatrixSBWT::create (include/matrix_sbwt.hpp:32-200) after
+ * the file read: rank structures, C array, device upload. Fails with
+ * SBWT_ERR_FORMAT unless C[4] == n. Builds the presearch table of
+ * cfg->presearch_depth. */
+int sbwt_index_from_rows(const uint64_t *const rows[4], const uint64_t *group,
+                         uint64_t n, uint32_t k, const sbwt_config *cfg,
+                         sbwt_index **out);
+
+/* Same from the reference's four bit-vector files ([u64 LE n][ceil(n/8)
+ * bytes], include/bit_vector.hpp:65-128); group_path may be NULL. Replaces
+ * MatrixSBWT::create including BitVector::create/read. */
+int sbwt_index_from_files(const char *const paths[4], const char *group_path,
+                          uint32_t k, const sbwt_config *cfg, sbwt_index **out);
+
+/* GPU SBWT construction from sequences (no reference counterpart: the
+ * reference reads bit vectors made by an external tool). d_seq holds the
+ * concatenated ASCII sequences, d_starts[nseq+1] their offsets; non-ACGT
+ * characters break k-mers. Builds the suffix-group-start row too. */
+int sbwt_index_build(const uint8_t *d_seq, const uint64_t *d_starts,
+                     uint64_t nseq, uint32_t k, const sbwt_config *cfg,
+                     void *stream, sbwt_index **out);
+
+/* Frees every device buffer (include/gpu/device_memory.hpp:81-121). */
+int sbwt_index_free(sbwt_index *ix);
+
+int sbwt_index_get_info(const sbwt_index *ix, sbwt_index_info *info);
+
+/* Copies the rows back to host memory, nwords words each (>= ceil(n/64)),
+ * MSB-first; group may be NULL. */
+int sbwt_index_get_rows(const sbwt_index *ix, uint64_t *const rows[4],
+                        uint64_t *group, uint64_t nwords);
+
+/* Writes the rows in the reference's bit-vector file format. */
+int sbwt_index_write_files(const sbwt_index *ix, const char *const paths[4],
+                           const char *group_path);
+
+/* (Re)builds the presearch table for m-mers, 0 <= m <= min(k, 14).
+ * Replaces epic::PreSearch::call_presearch (include/presearch.hpp:32-130)
+ * and the presearch kernel (include/gpu/presearch_kernel.hpp:19-37). */
+int sbwt_presearch(sbwt_index *ix, uint32_t m, void *stream);
+
+/* Host copies of the presearch table: 4^m intervals [left, right]. */
+int sbwt_presearch_get(const sbwt_index *ix, uint64_t *left, uint64_t *right);
+
+/* Re-lays the rank chunks (SBWT_LAYOUT_*), e.g. for the autotuner. */
+int sbwt_index_set_layout(sbwt_index *ix, uint32_t layout, void *stream);
+
+/* Searches count top-aligned k-mers (char 0 in bits 63:62, as the
+ * reference's kernel holds them after extraction, search_kernel.hpp:66-68);
+ * out[q] = node or ~0. */
+int sbwt_search_kmers(const sbwt_index *ix, const uint64_t *d_kmers,
+                      uint64_t count, uint64_t *d_out, void *stream);
+
+/* The reference kernel ABI (include/gpu/search_kernel.hpp:62-63, launched by
+ * launch_search_kernel, include/gpu/search_and_presearch_launchers.hpp:21):
+ * d_packed is the 2-bit string of sbwt_pack, d_positions holds count k-mer
+ * start offsets (in characters) and is overwritten in place by the results. */
+int sbwt_search_positions(const sbwt_index *ix, const uint64_t *d_packed,
+                          uint64_t *d_positions, uint64_t count, void *stream);
+
+/* Words of the packed string and of the N mask for n_chars characters. */
+uint64_t sbwt_packed_words(uint64_t n_chars);
+uint64_t sbwt_nmask_words(uint64_t n_chars);
+
+/* ASCII -> 2-bit, 32 characters per word, char 0 in bits 63:62, C/c=1,
+ * G/g=2, T/t=3, anything else 0 (include/gpu/create_kmer_string.hpp:22-42).
+ * Characters past n_chars read as '\0'. d_nmask (optional) gets one bit per
+ * character, MSB-first, set where the character is not A/C/G/T (any case). */
+int sbwt_pack(const uint8_t *d_chars, uint64_t n_chars, uint64_t *d_packed,
+              uint64_t *d_nmask, void *stream);
+
+/* d_offsets[rows+1]: exclusive scan of max(0, len_r - k + 1) over rows
+ * given by d_row_starts[rows+1] (rows shorter than k yield no k-mers). */
+int sbwt_kmer_offsets(const uint64_t *d_row_starts, uint64_t rows, uint32_t k,
+                      uint64_t *d_offsets, void *stream);
+
+/* Start offset of every k-mer of every row (include/gpu/positions_kernel.hpp:18-36);
+ * d_positions has d_offsets[rows] entries. */
+int sbwt_positions(const uint64_t *d_row_starts, const uint64_t *d_offsets,
+                   uint64_t rows, uint32_t k, uint64_t *d_positions,
+                   void *stream);
+
+/* Upper bound of the work items of sbwt_search_reads, and the work-item
+ * prefix d_work[rows+1] for segments of seg k-mers (0 = the index's
+ * configured segment length). */
+uint64_t sbwt_reads_work_bound(uint64_t rows, uint64_t total_kmers, uint32_t seg);
+int sbwt_reads_work(const uint64_t *d_offsets, uint64_t rows, uint32_t seg,
+                    uint64_t *d_work, void *stream);
+
+/* Searches every k-mer of every row; d_out[d_offsets[r] + j] is the answer
+ * for the k-mer starting at character j of row r. Replaces the query half of
+ * epic::KmerSearch::search (include/epicseq/kmer_search.hpp:97-153):
+ * packing, positions and SearchCaller::search fused into one pass over the
+ * rows. d_nmask may be NULL only with SBWT_READS_N_AS_A. work_bound is
+ * sbwt_reads_work_bound of the same rows. No host synchronisation; can be
+ * captured in a HIP graph. */
+int sbwt_search_reads(const sbwt_index *ix, const uint64_t *d_packed,
+                      const uint64_t *d_nmask, const uint64_t *d_row_starts,
+                      const uint64_t *d_offsets, const uint64_t *d_work,
+                      uint64_t rows, uint64_t work_bound, uint32_t seg,
+                      uint64_t *d_out, uint32_t flags, void *stream);
+
+/* Message of the calling thread's last failure ("" if none). */
+const char *sbwt_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

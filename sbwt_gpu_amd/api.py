@@ -1,0 +1,267 @@
+"""Host API over the C ABI, with torch tensors as device buffers.
+
+Mirrors the reference's operator interface for the search path:
+``Index.from_files`` / ``Index.from_rows`` stand for epic::MatrixSBWT::create
+(include/matrix_sbwt.hpp:32), ``Index.presearch`` for epic::PreSearch
+(include/presearch.hpp:32), ``Index.search_positions`` for the search kernel
+ABI (include/gpu/search_kernel.hpp:62-63), and ``Index.search_reads`` for the
+query half of epic::KmerSearch::search (include/epicseq/kmer_search.hpp:97-153).
+Unsigned 64-bit values travel in torch.int64 tensors (same bits); ~0
+(absent) reads as -1, the reference's printed value
+(include/epicseq/results_persister.hpp:88-92).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _capi
+from ._capi import check
+
+NONE = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def _stream(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def _ptr(t: torch.Tensor | None):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+def _as_u64_np(a) -> np.ndarray:
+    a = np.asarray(a)
+    if a.dtype == np.int64:
+        return a.view(np.uint64)
+    return np.ascontiguousarray(a, dtype=np.uint64)
+
+
+@dataclass
+class Config:
+    presearch_depth: int = 12
+    layout: int = 16
+    segment_kmers: int = 256
+
+    def to_c(self) -> _capi.Config:
+        c = _capi.Config()
+        check(_capi.load().sbwt_config_init(C.byref(c)))
+        c.presearch_depth = self.presearch_depth
+        c.layout = self.layout
+        c.segment_kmers = self.segment_kmers
+        return c
+
+
+class Index:
+    """An SBWT index resident in HBM (owned by the library)."""
+
+    def __init__(self, handle: C.c_void_p, config: Config):
+        self._h = handle
+        self.config = config
+        self.lib = _capi.load()
+
+    # ---- construction -------------------------------------------------
+    @classmethod
+    def from_rows(cls, rows, n: int, k: int, group=None, config: Config | None = None) -> "Index":
+        """rows: four host arrays of MSB-first u64 words (A, C, G, T)."""
+        config = config or Config()
+        L = _capi.load()
+        arrs = [np.ascontiguousarray(_as_u64_np(r)) for r in rows]
+        ptrs = (C.POINTER(C.c_uint64) * 4)(*[a.ctypes.data_as(C.POINTER(C.c_uint64)) for a in arrs])
+        g = None
+        if group is not None:
+            garr = np.ascontiguousarray(_as_u64_np(group))
+            g = C.c_void_p(garr.ctypes.data)
+        h = C.c_void_p()
+        check(L.sbwt_index_from_rows(ptrs, g, n, k, C.byref(config.to_c()), C.byref(h)))
+        return cls(h, config)
+
+    @classmethod
+    def from_files(cls, paths, k: int, group_path: str | None = None, config: Config | None = None) -> "Index":
+        config = config or Config()
+        L = _capi.load()
+        ps = (C.c_char_p * 4)(*[p.encode() for p in paths])
+        h = C.c_void_p()
+        check(L.sbwt_index_from_files(ps, group_path.encode() if group_path else None, k,
+                                      C.byref(config.to_c()), C.byref(h)))
+        return cls(h, config)
+
+    @classmethod
+    def build(cls, seqs, k: int, config: Config | None = None, starts=None) -> "Index":
+        """GPU SBWT construction. seqs: list of str/bytes, or a device uint8
+        tensor of concatenated characters with `starts` (device int64, nseq+1)."""
+        config = config or Config()
+        L = _capi.load()
+        if isinstance(seqs, torch.Tensor):
+            chars, st = seqs, starts
+        else:
+            chars, st = _upload_rows(seqs)
+        h = C.c_void_p()
+        check(L.sbwt_index_build(_ptr(chars), _ptr(st), st.numel() - 1, k, C.byref(config.to_c()),
+                                 _stream(), C.byref(h)))
+        torch.cuda.synchronize()
+        return cls(h, config)
+
+    def close(self) -> None:
+        if self._h:
+            self.lib.sbwt_index_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- inspection ------------------------------------------------------
+    @property
+    def info(self) -> dict:
+        i = _capi.IndexInfo()
+        check(self.lib.sbwt_index_get_info(self._h, C.byref(i)))
+        return {"n": i.n, "C": list(i.C), "device_bytes": i.device_bytes, "k": i.k,
+                "presearch_depth": i.presearch_depth, "layout": i.layout, "has_groups": bool(i.has_groups)}
+
+    @property
+    def n(self) -> int:
+        return self.info["n"]
+
+    @property
+    def k(self) -> int:
+        return self.info["k"]
+
+    def rows(self):
+        """(A, C, G, T, group) host copies, MSB-first words, ceil(n/4096)*64 words."""
+        n = self.n
+        nw = ((n + 4095) // 4096) * 64
+        out = [np.zeros(nw, np.uint64) for _ in range(5)]
+        ptrs = (C.POINTER(C.c_uint64) * 4)(*[a.ctypes.data_as(C.POINTER(C.c_uint64)) for a in out[:4]])
+        g = C.c_void_p(out[4].ctypes.data) if self.info["has_groups"] else None
+        check(self.lib.sbwt_index_get_rows(self._h, ptrs, g, nw))
+        return out
+
+    def write_files(self, paths, group_path: str | None = None) -> None:
+        ps = (C.c_char_p * 4)(*[p.encode() for p in paths])
+        check(self.lib.sbwt_index_write_files(self._h, ps, group_path.encode() if group_path else None))
+
+    # ---- presearch / layout ------------------------------------------------
+    def presearch(self, m: int) -> None:
+        check(self.lib.sbwt_presearch(self._h, m, _stream()))
+
+    def presearch_table(self):
+        m = self.info["presearch_depth"]
+        left = np.zeros(max(1, 1 << (2 * m)), np.uint64)
+        right = np.zeros_like(left)
+        check(self.lib.sbwt_presearch_get(self._h, C.c_void_p(left.ctypes.data), C.c_void_p(right.ctypes.data)))
+        return (left, right) if m else (left[:0], right[:0])
+
+    def set_layout(self, layout: int) -> None:
+        check(self.lib.sbwt_index_set_layout(self._h, layout, _stream()))
+
+    # ---- search ---------------------------------------------------------------
+    def search_kmers(self, kmers: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """kmers: device int64 tensor of top-aligned 2-bit k-mers."""
+        if out is None:
+            out = torch.empty_like(kmers)
+        check(self.lib.sbwt_search_kmers(self._h, _ptr(kmers), kmers.numel(), _ptr(out), _stream()))
+        return out
+
+    def search_positions(self, packed: torch.Tensor, positions: torch.Tensor) -> torch.Tensor:
+        """In place over `positions`, like the reference kernel."""
+        check(self.lib.sbwt_search_positions(self._h, _ptr(packed), _ptr(positions), positions.numel(), _stream()))
+        return positions
+
+    def search_reads(self, reads: "ReadBatch", out: torch.Tensor | None = None, n_as_a: bool = False,
+                     independent: bool = False) -> torch.Tensor:
+        if reads.k != self.k:
+            raise ValueError(f"reads prepared for k={reads.k}, index has k={self.k}")
+        if out is None:
+            out = torch.empty(max(reads.total_kmers, 1), dtype=torch.int64, device=reads.chars.device)
+        flags = (_capi.READS_N_AS_A if n_as_a else 0) | (_capi.READS_INDEPENDENT if independent else 0)
+        check(self.lib.sbwt_search_reads(self._h, _ptr(reads.packed), _ptr(reads.nmask), _ptr(reads.row_starts),
+                                         _ptr(reads.offsets), _ptr(reads.work), reads.rows, reads.work_bound,
+                                         reads.seg, _ptr(out), flags, _stream()))
+        return out[: reads.total_kmers]
+
+
+def _upload_rows(seqs, device="cuda"):
+    enc = [s.encode() if isinstance(s, str) else bytes(s) for s in seqs]
+    lens = np.array([len(s) for s in enc], dtype=np.int64)
+    starts = np.zeros(len(enc) + 1, dtype=np.int64)
+    starts[1:] = np.cumsum(lens)
+    buf = np.frombuffer(b"".join(enc), dtype=np.uint8) if starts[-1] else np.zeros(1, np.uint8)
+    chars = torch.from_numpy(buf.copy()).to(device)
+    return chars, torch.from_numpy(starts).to(device)
+
+
+class ReadBatch:
+    """A set of rows (reads) resident in HBM, prepared for the read search:
+    characters, row starts, packed 2-bit string + non-ACGT mask, per-row
+    k-mer offsets and the work prefix."""
+
+    def __init__(self, chars: torch.Tensor, row_starts: torch.Tensor, k: int, seg: int = 256,
+                 total_kmers: int | None = None, host_starts: np.ndarray | None = None):
+        L = _capi.load()
+        self.chars, self.row_starts, self.k, self.seg = chars, row_starts, k, seg
+        self.rows = row_starts.numel() - 1
+        n_chars = chars.numel()
+        dev = chars.device
+        self.packed = torch.empty(L.sbwt_packed_words(n_chars), dtype=torch.int64, device=dev)
+        self.nmask = torch.empty(L.sbwt_nmask_words(n_chars), dtype=torch.int64, device=dev)
+        self.offsets = torch.empty(self.rows + 1, dtype=torch.int64, device=dev)
+        s = _stream()
+        check(L.sbwt_pack(_ptr(chars), n_chars, _ptr(self.packed), _ptr(self.nmask), s))
+        check(L.sbwt_kmer_offsets(_ptr(row_starts), self.rows, k, _ptr(self.offsets), s))
+        if total_kmers is None:
+            if host_starts is not None:
+                lens = np.diff(host_starts.astype(np.int64))
+                total_kmers = int(np.maximum(lens - k + 1, 0).sum())
+            else:
+                total_kmers = int(self.offsets[-1].item())
+        self.total_kmers = total_kmers
+        self.work_bound = L.sbwt_reads_work_bound(self.rows, total_kmers, seg)
+        self.work = torch.empty(self.rows + 1, dtype=torch.int64, device=dev)
+        check(L.sbwt_reads_work(_ptr(self.offsets), self.rows, seg, _ptr(self.work), s))
+
+    @classmethod
+    def from_host(cls, seqs, k: int, seg: int = 256, device="cuda") -> "ReadBatch":
+        chars, starts = _upload_rows(seqs, device)
+        hs = starts.cpu().numpy()
+        return cls(chars, starts, k, seg, host_starts=hs)
+
+    def positions(self) -> torch.Tensor:
+        """Start offset of every k-mer (include/gpu/positions_kernel.hpp:18-36)."""
+        L = _capi.load()
+        out = torch.empty(max(self.total_kmers, 1), dtype=torch.int64, device=self.chars.device)
+        check(L.sbwt_positions(_ptr(self.row_starts), _ptr(self.offsets), self.rows, self.k, _ptr(out), _stream()))
+        return out[: self.total_kmers]
+
+
+def pack(chars: torch.Tensor):
+    """2-bit packing + non-ACGT mask of a device uint8 tensor
+    (include/gpu/create_kmer_string.hpp:22-42)."""
+    L = _capi.load()
+    n = chars.numel()
+    packed = torch.empty(L.sbwt_packed_words(n), dtype=torch.int64, device=chars.device)
+    nmask = torch.empty(L.sbwt_nmask_words(n), dtype=torch.int64, device=chars.device)
+    check(L.sbwt_pack(_ptr(chars), n, _ptr(packed), _ptr(nmask), _stream()))
+    return packed, nmask
+
+
+def to_u64(t: torch.Tensor) -> np.ndarray:
+    """Device int64 tensor -> host uint64 array (same bits)."""
+    return t.detach().cpu().numpy().view(np.uint64)

@@ -1,0 +1,47 @@
+// Internal definition of sbwt_index and the helpers shared by the HIP
+// sources of libsbwt_gpu_amd.
+#pragma once
+#include "layout.hpp"
+
+struct sbwt_index {
+  int device = 0;
+  sbwt::u32 k = 0;
+  sbwt::u64 n = 0;
+  sbwt::u64 C[5] = {0, 0, 0, 0, 0};
+  // rows A,C,G,T,group as MSB-first u64 words, row r at d_rows + r * nwords
+  sbwt::u64 nwords = 0;
+  sbwt::u64* d_rows = nullptr;
+  sbwt::u32 has_groups = 0;
+  // rank chunks of the current layout, 5 rows of row_q 16-byte units
+  sbwt::u32 layout = SBWT_LAYOUT_L16;
+  sbwt::u64 row_q = 0;
+  uint4* d_lines = nullptr;
+  // presearch table of 4^m intervals
+  sbwt::u32 m = 0;
+  uint2* d_pre = nullptr;
+  sbwt_config cfg{};
+  sbwt::u64 num_kmers = 0;  // |K| when built on the GPU, else 0
+};
+
+namespace sbwt {
+
+constexpr u32 kRows = 5;    // A, C, G, T, suffix-group starts
+constexpr u32 kGroupRow = 4;
+constexpr u32 kMaxPresearch = 14;
+
+inline u64 words_for(u64 n) { return ((n + 4095) / 4096) * 64; }
+
+DevIndex dev_view(const sbwt_index* ix);
+
+// Builds d_lines for ix->layout from d_rows and sets C from the device.
+int build_layout(sbwt_index* ix, hipStream_t s);
+// Finishes an index whose d_rows are filled: layout, C check, presearch.
+int finish_index(sbwt_index* ix, hipStream_t s);
+// Frees everything (safe on partially built indexes).
+void destroy_index(sbwt_index* ix);
+// A fresh index shell with cfg applied (rows not allocated).
+sbwt_index* new_index(const sbwt_config* cfg, u32 k, u64 n);
+// Validates n and k against what the device layout supports.
+int check_shape(u64 n, u32 k);
+
+}  // namespace sbwt

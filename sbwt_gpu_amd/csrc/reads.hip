@@ -1,0 +1,156 @@
+// Query ingest kernels of libsbwt_gpu_amd: 2-bit packing with a non-ACGT
+// mask, per-row k-mer offsets, k-mer start positions, and the work prefix
+// of the read search.
+//
+// Reference counterparts: create_kmer_string (include/gpu/create_kmer_string.hpp:22-42),
+// create_positions (include/gpu/positions_kernel.hpp:18-36) and the host-side
+// position counting of KmerSearch::read_file (include/epicseq/kmer_search.hpp:319-342).
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+#include <rocprim/iterator/transform_iterator.hpp>
+
+#include "index.hpp"
+
+namespace sbwt {
+
+// The reference packing table (create_kmer_string.hpp:20): C/c->1, G/g->2,
+// T/t->3, every other byte -> 0; `bad` marks bytes that are not A/C/G/T.
+__device__ __forceinline__ u32 code_of(u32 b, u32& bad) {
+  const u32 lc = b | 0x20u;
+  bad = (lc == 'a' || lc == 'c' || lc == 'g' || lc == 't') ? 0u : 1u;
+  return lc == 'c' ? 1u : (lc == 'g' ? 2u : (lc == 't' ? 3u : 0u));
+}
+
+// Thread q packs characters [32q, 32q+32): one packed word (char 0 in bits
+// 63:62) and one 32-bit half of the mask (MSB-first, so the half of mask
+// word q/2 that holds them is u32 q^1 in little-endian memory).
+__global__ __launch_bounds__(256) void k_pack(const u8* __restrict__ chars, u64 n, u64* __restrict__ packed,
+                                              u64 pwords, u32* __restrict__ mask32, u64 mhalves, u64 threads) {
+  const u64 q = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x;
+  if (q >= threads) return;
+  const u64 first = q * 32;
+  u32 b[32];
+  if (first + 32 <= n && (reinterpret_cast<uintptr_t>(chars + first) & 15) == 0) {
+    const uint4* p = reinterpret_cast<const uint4*>(chars + first);
+    const uint4 v0 = p[0], v1 = p[1];
+    const u32 w[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+#pragma unroll
+    for (int t = 0; t < 32; t++) b[t] = (w[t >> 2] >> (8 * (t & 3))) & 0xffu;
+  } else {
+#pragma unroll
+    for (int t = 0; t < 32; t++) b[t] = (first + t < n) ? chars[first + t] : 0u;
+  }
+  u64 word = 0;
+  u32 half = 0;
+#pragma unroll
+  for (int t = 0; t < 32; t++) {
+    u32 bad;
+    const u32 c = code_of(b[t], bad);
+    word |= static_cast<u64>(c) << (62 - 2 * t);
+    half |= bad << (31 - t);
+  }
+  if (q < pwords) packed[q] = word;
+  if (mask32 && (q ^ 1) < mhalves) mask32[q ^ 1] = half;
+}
+
+struct KmerCount {
+  const u64* rs;
+  u64 rows;
+  u32 k;
+  __host__ __device__ u64 operator()(u64 r) const {
+    if (r >= rows) return 0;
+    const u64 len = rs[r + 1] - rs[r];
+    return len >= k ? len - k + 1 : 0;
+  }
+};
+
+struct WorkCount {
+  const u64* off;
+  u64 rows;
+  u32 seg;
+  __host__ __device__ u64 operator()(u64 r) const {
+    if (r >= rows) return 0;
+    const u64 nk = off[r + 1] - off[r];
+    return (nk + seg - 1) / seg;
+  }
+};
+
+template <class F>
+static int scan_rows(F f, u64 rows, u64* out, hipStream_t s) {
+  auto it = rocprim::make_transform_iterator(rocprim::make_counting_iterator<u64>(0), f);
+  size_t tmp = 0;
+  SBWT_HIP(rocprim::exclusive_scan(nullptr, tmp, it, out, u64(0), rows + 1, rocprim::plus<u64>(), s));
+  void* d_tmp = nullptr;
+  SBWT_HIP(hipMallocAsync(&d_tmp, tmp ? tmp : 1, s));
+  hipError_t e = rocprim::exclusive_scan(d_tmp, tmp, it, out, u64(0), rows + 1, rocprim::plus<u64>(), s);
+  (void)hipFreeAsync(d_tmp, s);
+  if (e != hipSuccess) return fail(SBWT_ERR_HIP, "row scan failed: %s", hipGetErrorString(e));
+  return SBWT_OK;
+}
+
+// One 64-lane block per row writes its consecutive start offsets.
+__global__ __launch_bounds__(64) void k_positions(const u64* __restrict__ rs, const u64* __restrict__ off, u64 rows,
+                                                  u64* __restrict__ out) {
+  for (u64 r = blockIdx.x; r < rows; r += gridDim.x) {
+    const u64 nk = off[r + 1] - off[r];
+    const u64 base = rs[r];
+    u64* o = out + off[r];
+    for (u64 i = threadIdx.x; i < nk; i += blockDim.x) o[i] = base + i;
+  }
+}
+
+}  // namespace sbwt
+
+using namespace sbwt;
+
+extern "C" {
+
+uint64_t sbwt_packed_words(uint64_t n_chars) { return n_chars / 32 + 2; }
+uint64_t sbwt_nmask_words(uint64_t n_chars) { return n_chars / 64 + 2; }
+
+int sbwt_pack(const uint8_t* d_chars, uint64_t n_chars, uint64_t* d_packed, uint64_t* d_nmask, void* stream) {
+  clear_error();
+  SBWT_REQUIRE(d_packed, "d_packed must not be NULL");
+  SBWT_REQUIRE(d_chars || n_chars == 0, "d_chars must not be NULL");
+  const u64 pw = sbwt_packed_words(n_chars);
+  const u64 mh = 2 * sbwt_nmask_words(n_chars);
+  const u64 threads = n_chars / 32 + 4;  // covers every packed word and mask half
+  const u32 bt = 256;
+  k_pack<<<(threads + bt - 1) / bt, bt, 0, as_stream(stream)>>>(d_chars, n_chars, d_packed, pw,
+                                                               reinterpret_cast<u32*>(d_nmask), mh, threads);
+  SBWT_HIP(hipGetLastError());
+  return SBWT_OK;
+}
+
+int sbwt_kmer_offsets(const uint64_t* d_row_starts, uint64_t rows, uint32_t k, uint64_t* d_offsets, void* stream) {
+  clear_error();
+  SBWT_REQUIRE(d_row_starts && d_offsets, "device buffers must not be NULL");
+  SBWT_REQUIRE(k >= 1 && k <= 32, "k = %u: must be in [1, 32]", k);
+  return scan_rows(KmerCount{d_row_starts, rows, k}, rows, d_offsets, as_stream(stream));
+}
+
+int sbwt_positions(const uint64_t* d_row_starts, const uint64_t* d_offsets, uint64_t rows, uint32_t k,
+                   uint64_t* d_positions, void* stream) {
+  clear_error();
+  (void)k;
+  if (rows == 0) return SBWT_OK;
+  SBWT_REQUIRE(d_row_starts && d_offsets && d_positions, "device buffers must not be NULL");
+  const u64 grid = rows < (1u << 20) ? rows : (1u << 20);
+  k_positions<<<grid, 64, 0, as_stream(stream)>>>(d_row_starts, d_offsets, rows, d_positions);
+  SBWT_HIP(hipGetLastError());
+  return SBWT_OK;
+}
+
+uint64_t sbwt_reads_work_bound(uint64_t rows, uint64_t total_kmers, uint32_t seg) {
+  if (seg == 0) seg = 256;
+  return rows + total_kmers / seg + 1;
+}
+
+int sbwt_reads_work(const uint64_t* d_offsets, uint64_t rows, uint32_t seg, uint64_t* d_work, void* stream) {
+  clear_error();
+  SBWT_REQUIRE(d_offsets && d_work, "device buffers must not be NULL");
+  if (seg == 0) seg = 256;
+  return scan_rows(WorkCount{d_offsets, rows, seg}, rows, d_work, as_stream(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,238 @@
+// Search kernels of libsbwt_gpu_amd.
+//
+// The reference runs one thread per k-mer: a presearch gather, then up to
+// k-m narrowing steps of two ranks each, early exit on an empty interval
+// (include/gpu/search_kernel.hpp:61-104). Here every rank is one chunk read
+// (layout.hpp) and the two chunk reads of a step are issued together. The
+// read search adds streaming: once the k-mer x at node j is found, the next
+// k-mer x[1..k-1]c is found from j in one step, because the suffix-group
+// start g of j (the first node sharing x's last k-1 characters) carries the
+// only c-edge into it: the answer is C[c] + rank_c(g) if g has a c-edge,
+// else absent. This yields the same node ids as the full search.
+#include "index.hpp"
+
+namespace sbwt {
+
+// Full search of a top-aligned k-mer (char 0 in bits 63:62).
+template <int CB>
+__device__ __forceinline__ u64 full_search(const DevIndex& ix, u64 kmer) {
+  u32 l, r, i;
+  if (ix.m) {
+    const uint2 e = ix.pre[kmer >> (64 - 2 * ix.m)];
+    l = e.x;
+    r = e.y;
+    i = ix.m;
+  } else {
+    l = 0;
+    r = ix.C[4] - 1u;
+    i = 0;
+  }
+  for (; i < ix.k; i++) {
+    const u32 c = static_cast<u32>(kmer >> (62 - 2 * i)) & 3u;
+    narrow<CB>(ix, c, l, r);
+    if (l > r) break;
+  }
+  return l > r ? kNone : static_cast<u64>(l);
+}
+
+// Next k-mer after a found one at node j, appending character c.
+template <int CB>
+__device__ __forceinline__ u64 stream_step(const DevIndex& ix, u32 j, u32 c) {
+  constexpr u32 COLS = Layout<CB>::COLS;
+  const uint4* row = row_of(ix, c);
+  u32 ch = j / COLS;
+  u32 o = j - ch * COLS;
+  Chunk<CB> q;
+  load_chunk<CB>(row, ch, q);
+  // a set bit means j starts its group and has the c-edge
+  if (bit_chunk<CB>(q, o)) return static_cast<u64>(c_of(ix, c) + rank_chunk<CB>(q, o));
+  // otherwise find the group start g <= j; column 0 always starts a group
+  const uint4* grow = row_of(ix, kGroupRow);
+  Chunk<CB> gq;
+  load_chunk<CB>(grow, ch, gq);
+  int p = last_one_upto<CB>(gq, o);
+  while (p < 0 && ch > 0) {
+    ch--;
+    load_chunk<CB>(grow, ch, gq);
+    p = last_one_upto<CB>(gq, COLS - 1);
+  }
+  const u32 g = p < 0 ? 0u : ch * COLS + static_cast<u32>(p);
+  if (g == j) return kNone;
+  const u32 gch = g / COLS, go = g - gch * COLS;
+  load_chunk<CB>(row, gch, q);
+  if (!bit_chunk<CB>(q, go)) return kNone;
+  return static_cast<u64>(c_of(ix, c) + rank_chunk<CB>(q, go));
+}
+
+// The reference's two-word extraction (search_kernel.hpp:66-68): the 64
+// bits starting at bit 2*pos of the packed string.
+__device__ __forceinline__ u64 extract2(const u64* __restrict__ w, u64 pos) {
+  const u64 b = pos << 1;
+  return (w[b >> 6] << (b & 63)) | ((w[(b >> 6) + 1] >> (63 - (b & 63))) >> 1);
+}
+
+// The 64 bits of a one-bit-per-character mask starting at character pos.
+__device__ __forceinline__ u64 extract1(const u64* __restrict__ w, u64 pos) {
+  return (w[pos >> 6] << (pos & 63)) | ((w[(pos >> 6) + 1] >> (63 - (pos & 63))) >> 1);
+}
+
+template <int CB>
+__global__ __launch_bounds__(256) void k_search_kmers(DevIndex ix, const u64* __restrict__ kmers, u64 count,
+                                                      u64* __restrict__ out) {
+  const u64 q = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x;
+  if (q >= count) return;
+  out[q] = full_search<CB>(ix, kmers[q]);
+}
+
+template <int CB>
+__global__ __launch_bounds__(256) void k_search_positions(DevIndex ix, const u64* __restrict__ packed,
+                                                          u64* __restrict__ pos_res, u64 count) {
+  const u64 q = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x;
+  if (q >= count) return;
+  pos_res[q] = full_search<CB>(ix, extract2(packed, pos_res[q]));
+}
+
+// One lane per work item: up to `seg` consecutive k-mers of one row.
+template <int CB, bool STREAM, bool NBREAK>
+__global__ __launch_bounds__(256) void k_search_reads(DevIndex ix, const u64* __restrict__ packed,
+                                                      const u64* __restrict__ nmask,
+                                                      const u64* __restrict__ row_starts,
+                                                      const u64* __restrict__ offsets,
+                                                      const u64* __restrict__ work, u64 rows, u64 bound, u32 seg,
+                                                      u64* __restrict__ out) {
+  const u64 gid = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x;
+  if (gid >= bound || rows == 0) return;
+  if (gid >= work[rows]) return;
+  // the row: work[r] <= gid < work[r+1]
+  u64 lo = 0, hi = rows;
+  while (hi - lo > 1) {
+    const u64 mid = (lo + hi) >> 1;
+    if (work[mid] <= gid) lo = mid; else hi = mid;
+  }
+  const u64 r = lo;
+  const u64 nk = offsets[r + 1] - offsets[r];
+  const u64 j0 = (gid - work[r]) * seg;
+  const u64 cnt = (nk - j0) < seg ? (nk - j0) : seg;
+  const u64 c0 = row_starts[r] + j0;
+  u64* o = out + offsets[r] + j0;
+  const u32 k = ix.k;
+  bool prev = false;
+  u32 pnode = 0;
+  for (u64 j = 0; j < cnt; j++) {
+    const u64 pos = c0 + j;
+    const u64 kmer = extract2(packed, pos);
+    u64 res;
+    if (NBREAK && (extract1(nmask, pos) >> (64 - k)) != 0) {
+      res = kNone;
+    } else if (STREAM && prev) {
+      res = stream_step<CB>(ix, pnode, static_cast<u32>(kmer >> (64 - 2 * k)) & 3u);
+    } else {
+      res = full_search<CB>(ix, kmer);
+    }
+    prev = res != kNone;
+    pnode = static_cast<u32>(res);
+    o[j] = res;
+  }
+}
+
+template <int CB>
+static int launch_kmers(const sbwt_index* ix, const u64* kmers, u64 count, u64* out, hipStream_t s) {
+  const u32 bt = 256;
+  k_search_kmers<CB><<<(count + bt - 1) / bt, bt, 0, s>>>(dev_view(ix), kmers, count, out);
+  SBWT_HIP(hipGetLastError());
+  return SBWT_OK;
+}
+
+template <int CB>
+static int launch_positions(const sbwt_index* ix, const u64* packed, u64* pos, u64 count, hipStream_t s) {
+  const u32 bt = 256;
+  k_search_positions<CB><<<(count + bt - 1) / bt, bt, 0, s>>>(dev_view(ix), packed, pos, count);
+  SBWT_HIP(hipGetLastError());
+  return SBWT_OK;
+}
+
+template <int CB, bool STREAM, bool NBREAK>
+static int launch_reads3(const sbwt_index* ix, const u64* packed, const u64* nmask, const u64* rs, const u64* off,
+                         const u64* work, u64 rows, u64 bound, u32 seg, u64* out, hipStream_t s) {
+  const u32 bt = 256;
+  k_search_reads<CB, STREAM, NBREAK>
+      <<<(bound + bt - 1) / bt, bt, 0, s>>>(dev_view(ix), packed, nmask, rs, off, work, rows, bound, seg, out);
+  SBWT_HIP(hipGetLastError());
+  return SBWT_OK;
+}
+
+template <int CB>
+static int launch_reads(const sbwt_index* ix, bool stream, bool nbreak, const u64* packed, const u64* nmask,
+                        const u64* rs, const u64* off, const u64* work, u64 rows, u64 bound, u32 seg, u64* out,
+                        hipStream_t s) {
+  if (stream && nbreak) return launch_reads3<CB, true, true>(ix, packed, nmask, rs, off, work, rows, bound, seg, out, s);
+  if (stream) return launch_reads3<CB, true, false>(ix, packed, nmask, rs, off, work, rows, bound, seg, out, s);
+  if (nbreak) return launch_reads3<CB, false, true>(ix, packed, nmask, rs, off, work, rows, bound, seg, out, s);
+  return launch_reads3<CB, false, false>(ix, packed, nmask, rs, off, work, rows, bound, seg, out, s);
+}
+
+
+}  // namespace sbwt
+
+using namespace sbwt;
+
+extern "C" {
+
+int sbwt_search_kmers(const sbwt_index* ix, const uint64_t* d_kmers, uint64_t count, uint64_t* d_out,
+                      void* stream) {
+  clear_error();
+  SBWT_REQUIRE(ix, "index is NULL");
+  if (count == 0) return SBWT_OK;
+  SBWT_REQUIRE(d_kmers && d_out, "device buffers must not be NULL");
+  hipStream_t s = as_stream(stream);
+  switch (ix->layout) {
+    case 16: return launch_kmers<16>(ix, d_kmers, count, d_out, s);
+    case 32: return launch_kmers<32>(ix, d_kmers, count, d_out, s);
+    default: return launch_kmers<64>(ix, d_kmers, count, d_out, s);
+  }
+}
+
+int sbwt_search_positions(const sbwt_index* ix, const uint64_t* d_packed, uint64_t* d_positions, uint64_t count,
+                          void* stream) {
+  clear_error();
+  SBWT_REQUIRE(ix, "index is NULL");
+  if (count == 0) return SBWT_OK;
+  SBWT_REQUIRE(d_packed && d_positions, "device buffers must not be NULL");
+  hipStream_t s = as_stream(stream);
+  switch (ix->layout) {
+    case 16: return launch_positions<16>(ix, d_packed, d_positions, count, s);
+    case 32: return launch_positions<32>(ix, d_packed, d_positions, count, s);
+    default: return launch_positions<64>(ix, d_packed, d_positions, count, s);
+  }
+}
+
+int sbwt_search_reads(const sbwt_index* ix, const uint64_t* d_packed, const uint64_t* d_nmask,
+                      const uint64_t* d_row_starts, const uint64_t* d_offsets, const uint64_t* d_work, uint64_t rows,
+                      uint64_t work_bound, uint32_t seg, uint64_t* d_out, uint32_t flags, void* stream) {
+  clear_error();
+  SBWT_REQUIRE(ix, "index is NULL");
+  if (rows == 0 || work_bound == 0) return SBWT_OK;
+  SBWT_REQUIRE(d_packed && d_row_starts && d_offsets && d_work && d_out, "device buffers must not be NULL");
+  const bool nbreak = !(flags & SBWT_READS_N_AS_A);
+  const bool stream_mode = !(flags & SBWT_READS_INDEPENDENT);
+  SBWT_REQUIRE(!nbreak || d_nmask, "d_nmask is required unless SBWT_READS_N_AS_A is set");
+  if (stream_mode && !ix->has_groups)
+    return fail(SBWT_ERR_UNSUPPORTED,
+                "streaming search needs the suffix-group row; open the index with it or pass "
+                "SBWT_READS_INDEPENDENT");
+  if (seg == 0) seg = 256;
+  hipStream_t s = as_stream(stream);
+  switch (ix->layout) {
+    case 16:
+      return launch_reads<16>(ix, stream_mode, nbreak, d_packed, d_nmask, d_row_starts, d_offsets, d_work, rows,
+                              work_bound, seg, d_out, s);
+    case 32:
+      return launch_reads<32>(ix, stream_mode, nbreak, d_packed, d_nmask, d_row_starts, d_offsets, d_work, rows,
+                              work_bound, seg, d_out, s);
+    default:
+      return launch_reads<64>(ix, stream_mode, nbreak, d_packed, d_nmask, d_row_starts, d_offsets, d_work, rows,
+                              work_bound, seg, d_out, s);
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,337 @@
+"""Bit-exact parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Every answer is an integer node id or ~0, so every comparison is exact."""
+import itertools
+import os
+
+import numpy as np
+import pytest
+
+import brute
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():  # pragma: no cover - CPU container
+    pytest.skip("needs a GPU", allow_module_level=True)
+
+import sbwt_gpu_amd as sg  # noqa: E402
+
+NONE = np.uint64(0xFFFFFFFFFFFFFFFF)
+ACGT = np.array(list("ACGT"))
+
+
+def genome(rng, n, p=(0.25, 0.25, 0.25, 0.25)):
+    return "".join(ACGT[rng.choice(4, size=n, p=p)])
+
+
+def dev_u64(a: np.ndarray) -> "torch.Tensor":
+    return torch.from_numpy(np.ascontiguousarray(a, np.uint64).view(np.int64)).cuda()
+
+
+def oracle_index(s, m, sb=1024):
+    ix = O.Index(s.rows, s.n, sb=sb)
+    ix.presearch(m)
+    return ix
+
+
+def topk(kmers_str):
+    return np.array([brute.encode_top(x) for x in kmers_str], np.uint64)
+
+
+def sample_queries(rng, g, k, count):
+    pos = rng.integers(0, len(g) - k + 1, count // 2)
+    sampled = [g[p:p + k] for p in pos]
+    rnd = ["".join(ACGT[rng.integers(0, 4, k)]) for _ in range(count - count // 2)]
+    q = sampled + rnd
+    rng.shuffle(q)
+    return q
+
+
+# ---------------------------------------------------------------------------
+# index upload, C array, presearch table
+
+
+@pytest.mark.parametrize("layout", [16, 32, 64])
+def test_index_info_and_presearch_table(layout):
+    rng = np.random.default_rng(5)
+    g = genome(rng, 3000)
+    k = 9
+    s = O.SBWT([g], k)
+    for m in (1, 4, 7):
+        ix = sg.Index.from_rows(s.rows, s.n, k, group=s.group, config=sg.Config(presearch_depth=m, layout=layout))
+        info = ix.info
+        assert info["n"] == s.n and info["C"] == list(map(int, s.C)) and info["layout"] == layout
+        left, right = ix.presearch_table()
+        ol, orr = oracle_index(s, m).presearch(m)
+        np.testing.assert_array_equal(left, ol)
+        np.testing.assert_array_equal(right, orr)
+        ix.close()
+
+
+@pytest.mark.parametrize("seqs,k", [(["ACGTACGTTGCAAGGCT", "TTGACCA"], 4), (["GATTACA", "CATTAG", "ACGTGTCA"], 3)])
+def test_every_kmer_tiny_against_brute_force(seqs, k):
+    order, _, _ = brute.sbwt(seqs, k)
+    K = brute.kmer_set(seqs, k)
+    s = O.SBWT(seqs, k)
+    allk = ["".join(p) for p in itertools.product("ACGT", repeat=k)]
+    want = np.array([brute.search(order, K, x) for x in allk], np.uint64)
+    q = dev_u64(topk(allk))
+    for layout in (16, 32, 64):
+        for m in range(0, k + 1):
+            ix = sg.Index.from_rows(s.rows, s.n, k, group=s.group, config=sg.Config(presearch_depth=m, layout=layout))
+            got = sg.to_u64(ix.search_kmers(q))
+            np.testing.assert_array_equal(got, want, err_msg=f"layout={layout} m={m}")
+            ix.close()
+
+
+# ---------------------------------------------------------------------------
+# search_kmers / search_positions against the oracle on the CPU-runnable case
+
+
+@pytest.mark.parametrize("layout,m", [(16, 8), (32, 8), (64, 8), (16, 0), (16, 12), (32, 5)])
+def test_search_kmers_one_mbp_scale(layout, m):
+    rng = np.random.default_rng(100 + layout + m)
+    g = genome(rng, 300_000)
+    k = 31
+    s = O.SBWT([g], k)
+    q = topk(sample_queries(rng, g, k, 40_000))
+    want = oracle_index(s, m).search(q, k)
+    ix = sg.Index.from_rows(s.rows, s.n, k, group=s.group, config=sg.Config(presearch_depth=m, layout=layout))
+    got = sg.to_u64(ix.search_kmers(dev_u64(q)))
+    np.testing.assert_array_equal(got, want)
+    assert (got != NONE).sum() >= 20_000
+
+
+def test_random_rows_stress_rank():
+    """Rows of random bits (total ones n-1, so C[4] = n): not an SBWT, but the
+    narrowing is still well defined, and random k-mers drive rank through
+    every chunk offset and chunk boundary of every layout."""
+    rng = np.random.default_rng(9)
+    n = 200_003
+    col = rng.integers(0, 4, n - 1)  # one 1-bit per column for n-1 columns
+    perm = rng.permutation(n)[: n - 1]
+    rows = [np.zeros(((n + 4095) // 4096) * 64, np.uint64) for _ in range(4)]
+    for c in range(4):
+        pos = np.sort(perm[col == c])
+        bits = np.zeros(len(rows[c]) * 64, np.uint8)
+        bits[pos] = 1
+        rows[c] = np.packbits(bits).view(">u8").astype(np.uint64)
+    k = 12
+    q = rng.integers(0, 2**63, 100_000, dtype=np.uint64) * np.uint64(2)
+    for m in (0, 3, 6):
+        oix = O.Index(rows, n)
+        oix.presearch(m)
+        want = oix.search(q, k)
+        for layout in (16, 32, 64):
+            ix = sg.Index.from_rows(rows, n, k, config=sg.Config(presearch_depth=m, layout=layout))
+            got = sg.to_u64(ix.search_kmers(dev_u64(q)))
+            np.testing.assert_array_equal(got, want, err_msg=f"m={m} layout={layout}")
+            ix.close()
+
+
+def test_search_positions_reference_abi():
+    rng = np.random.default_rng(3)
+    g = genome(rng, 100_000)
+    k = 31
+    s = O.SBWT([g], k)
+    reads = [g[p:p + 150] if i % 3 else "".join(ACGT[rng.integers(0, 4, 100)])
+             for i, p in enumerate(rng.integers(0, len(g) - 150, 500))]
+    chars, rs = O.parse_lines("\n".join(reads).encode())
+    raw = bytes(chars) + b"\0" * (32 - len(chars) % 32 + 32)
+    packed_ref = O.pack(raw[: (len(raw) // 32) * 32])
+    pos = O.positions(rs, k)
+    oix = oracle_index(s, 10)
+    want = oix.search(np.array([O.extract(packed_ref, int(p)) for p in pos], np.uint64), k)
+    ix = sg.Index.from_rows(s.rows, s.n, k, group=s.group, config=sg.Config(presearch_depth=10))
+    dchars = torch.from_numpy(np.frombuffer(bytes(chars), np.uint8).copy()).cuda()
+    packed, _ = sg.pack(dchars)
+    m = min(packed.numel(), packed_ref.size)
+    np.testing.assert_array_equal(sg.to_u64(packed)[:m], packed_ref[:m])
+    batch = sg.ReadBatch(dchars, dev_u64(rs), k, host_starts=rs)
+    dpos = batch.positions()
+    np.testing.assert_array_equal(sg.to_u64(dpos), pos)
+    got = sg.to_u64(ix.search_positions(packed, dpos))
+    np.testing.assert_array_equal(got, want)
+
+
+# ---------------------------------------------------------------------------
+# read search: streaming / independent, N policies, ragged rows
+
+
+def _reads(rng, g, n_reads, err=0.01, nrate=0.0, lens=(150, 150)):
+    out = []
+    for _ in range(n_reads):
+        L = int(rng.integers(lens[0], lens[1] + 1))
+        if L >= len(g):
+            L = len(g) - 1
+        p = int(rng.integers(0, len(g) - L))
+        r = np.array(list(g[p:p + L]))
+        flip = rng.random(L) < err
+        r[flip] = ACGT[rng.integers(0, 4, flip.sum())]
+        nn = rng.random(L) < nrate
+        r[nn] = "N"
+        out.append("".join(r))
+    return out
+
+
+@pytest.mark.parametrize("layout", [16, 32, 64])
+@pytest.mark.parametrize("seg", [256, 7, 1])
+def test_search_reads_matches_oracle(layout, seg):
+    rng = np.random.default_rng(layout * 10 + seg)
+    g = genome(rng, 200_000, (0.3, 0.2, 0.2, 0.3))
+    k = 31
+    s = O.SBWT([g], k)
+    reads = _reads(rng, g, 400, err=0.01, nrate=0.02, lens=(20, 400))
+    reads += ["", "ACGT", "N" * 40, g[:31], g[:32] + "N" + g[33:80]]
+    chars, rs = O.parse_lines("\n".join(reads).encode())
+    oix = oracle_index(s, 10)
+    ix = sg.Index.from_rows(s.rows, s.n, k, group=s.group, config=sg.Config(presearch_depth=10, layout=layout))
+    batch = sg.ReadBatch(torch.from_numpy(np.frombuffer(bytes(chars), np.uint8).copy()).cuda(), dev_u64(rs), k,
+                         seg=seg, host_starts=rs)
+    for n_as_a in (False, True):
+        want = O.search_reads(oix, chars, rs, k, n_breaks=not n_as_a)
+        for independent in (False, True):
+            got = sg.to_u64(ix.search_reads(batch, n_as_a=n_as_a, independent=independent))
+            np.testing.assert_array_equal(got, want, err_msg=f"n_as_a={n_as_a} independent={independent}")
+
+
+def test_streaming_needs_group_row():
+    rng = np.random.default_rng(2)
+    g = genome(rng, 5000)
+    s = O.SBWT([g], 15)
+    ix = sg.Index.from_rows(s.rows, s.n, 15, config=sg.Config(presearch_depth=6))
+    batch = sg.ReadBatch.from_host([g[:200]], 15)
+    with pytest.raises(sg.SbwtError):
+        ix.search_reads(batch)
+    got = sg.to_u64(ix.search_reads(batch, independent=True))
+    assert (got != NONE).all()
+
+
+def test_segmental_copies_groups():
+    """Repeats create suffix groups larger than one node, which exercises the
+    backward group-start scan of the streaming step."""
+    rng = np.random.default_rng(17)
+    base = genome(rng, 20_000)
+    parts = [base]
+    for _ in range(30):
+        p = int(rng.integers(0, 19_000))
+        seg = np.array(list(base[p:p + 800]))
+        flip = rng.random(800) < 0.02
+        seg[flip] = ACGT[rng.integers(0, 4, flip.sum())]
+        parts.append("".join(seg))
+    g = "".join(parts)
+    k = 21
+    s = O.SBWT([g], k)
+    groups = np.unpackbits(s.group.view(np.uint8).reshape(-1, 8)[:, ::-1].reshape(-1))[: s.n]
+    assert groups.sum() < s.n  # some groups have more than one node
+    reads = _reads(rng, g, 300, err=0.005, lens=(50, 300))
+    chars, rs = O.parse_lines("\n".join(reads).encode())
+    oix = oracle_index(s, 8)
+    want = O.search_reads(oix, chars, rs, k)
+    for layout in (16, 32, 64):
+        ix = sg.Index.from_rows(s.rows, s.n, k, group=s.group, config=sg.Config(presearch_depth=8, layout=layout))
+        batch = sg.ReadBatch.from_host(reads, k)
+        got = sg.to_u64(ix.search_reads(batch))
+        np.testing.assert_array_equal(got, want)
+
+
+# ---------------------------------------------------------------------------
+# GPU SBWT builder against the oracle's builder
+
+
+@pytest.mark.parametrize("seqs,k", [(["ACGTACGTTGCA"], 3), (["AAAAAAA"], 3), (["ACGTNACGTTTACG", "acgtacgt"], 3),
+                                    (["TTTT", "AC"], 5), (["CGTAC" * 4], 5), (["ACGT" * 10, "", "GGGGCCCC"], 1),
+                                    (["ACGTTGCA" * 9], 32)])
+def test_gpu_builder_tiny(seqs, k):
+    ref = O.SBWT(seqs, k)
+    ix = sg.Index.build(seqs, k, sg.Config(presearch_depth=min(k, 3)))
+    rows = ix.rows()
+    assert ix.n == ref.n
+    for c in range(4):
+        np.testing.assert_array_equal(rows[c][: ref.nwords], ref.rows[c])
+    np.testing.assert_array_equal(rows[4][: ref.nwords], ref.group)
+
+
+@pytest.mark.parametrize("k", [31, 32, 17])
+def test_gpu_builder_multi_sequence(k):
+    rng = np.random.default_rng(k)
+    seqs = [genome(rng, int(rng.integers(0, 3000))) for _ in range(40)]
+    seqs += [genome(rng, 50_000, (0.3, 0.2, 0.2, 0.3))]
+    seqs[3] = seqs[3][:100] + "NNN" + seqs[3][100:]
+    ref = O.SBWT(seqs, k)
+    ix = sg.Index.build(seqs, k, sg.Config(presearch_depth=8))
+    rows = ix.rows()
+    assert ix.n == ref.n and ix.info["C"] == list(map(int, ref.C))
+    for c in range(4):
+        np.testing.assert_array_equal(rows[c][: ref.nwords], ref.rows[c])
+    np.testing.assert_array_equal(rows[4][: ref.nwords], ref.group)
+
+
+# ---------------------------------------------------------------------------
+# files
+
+
+def test_files_round_trip(tmp_path):
+    rng = np.random.default_rng(4)
+    g = genome(rng, 30_000)
+    k = 25
+    s = O.SBWT([g], k)
+    paths = [str(tmp_path / f"{c}.bin") for c in "ACGT"]
+    for c in range(4):
+        O.bv_write(paths[c], s.rows[c], s.n)
+    O.bv_write(str(tmp_path / "grp.bin"), s.group, s.n)
+    ix = sg.Index.from_files(paths, k, group_path=str(tmp_path / "grp.bin"), config=sg.Config(presearch_depth=9))
+    assert ix.info["C"] == list(map(int, s.C))
+    out = [str(tmp_path / f"o{c}.bin") for c in "ACGT"]
+    ix.write_files(out)
+    for c in range(4):
+        assert open(out[c], "rb").read() == open(paths[c], "rb").read()
+    q = topk(sample_queries(rng, g, k, 5000))
+    np.testing.assert_array_equal(sg.to_u64(ix.search_kmers(dev_u64(q))), oracle_index(s, 9).search(q, k))
+
+
+def test_file_errors(tmp_path):
+    rng = np.random.default_rng(4)
+    s = O.SBWT([genome(rng, 2000)], 11)
+    paths = [str(tmp_path / f"{c}.bin") for c in "ACGT"]
+    for c in range(4):
+        O.bv_write(paths[c], s.rows[c], s.n)
+    with pytest.raises(sg.SbwtError, match="can not be opened"):
+        sg.Index.from_files(paths[:3] + [str(tmp_path / "missing.bin")], 11)
+    O.bv_write(paths[2], s.rows[2], s.n - 1)
+    with pytest.raises(sg.SbwtError, match="differ in length"):
+        sg.Index.from_files(paths, 11)
+    # swapping two rows keeps n but breaks nothing detectable; dropping a
+    # 1-bit breaks C[4] = n
+    bad = s.rows[0].copy()
+    nz = np.flatnonzero(bad)[0]
+    bad[nz] = np.uint64(0)
+    with pytest.raises(sg.SbwtError, match="not an SBWT"):
+        sg.Index.from_rows([bad, s.rows[1], s.rows[2], s.rows[3]], s.n, 11)
+
+
+# ---------------------------------------------------------------------------
+# scale: size-independent properties on a GPU-built index
+
+
+def test_scale_properties_streaming_equals_independent():
+    rng = np.random.default_rng(2024)
+    g = genome(rng, 5_000_000, (0.3, 0.2, 0.2, 0.3))
+    k = 31
+    ix = sg.Index.build([g], k, sg.Config(presearch_depth=12))
+    reads = _reads(rng, g, 20_000, err=0.01, lens=(150, 150))
+    batch = sg.ReadBatch.from_host(reads, k)
+    a = sg.to_u64(ix.search_reads(batch))
+    b = sg.to_u64(ix.search_reads(batch, independent=True))
+    np.testing.assert_array_equal(a, b)
+    # error-free reads are found everywhere, at distinct-or-equal nodes
+    clean = sg.ReadBatch.from_host([g[p:p + 150] for p in rng.integers(0, len(g) - 150, 2000)], k)
+    c = sg.to_u64(ix.search_reads(clean))
+    assert (c != NONE).all()
+    # the answers agree with search_kmers on the same k-mers
+    chars, rs = O.parse_lines("\n".join(reads[:2000]).encode())
+    km, valid = O.read_kmers(chars, rs, k)
+    d = sg.to_u64(ix.search_kmers(dev_u64(km)))
+    np.testing.assert_array_equal(d, a[: d.size])
