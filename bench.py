@@ -1,0 +1,236 @@
+#!/usr/bin/env python3
+"""Benchmark: k-mer membership queries/s of the SBWT read search on one MI355X.
+
+Workload (the plain GPU search at scale): a seeded 100 Mbp synthetic genome
+(GC 40%, 10% of it segmental copies of 1-50 kbp source segments at 1%
+substitutions), k = 31, presearch depth 12; 1,000,000 reads of 150 bp drawn
+from it with 1% substitutions, queried as all consecutive 31-mers
+(120,000,000 k-mers). The index is built on the GPU before timing; the reads
+are resident in HBM as ASCII when the timed region starts. One step = the
+whole query path over the batch: 2-bit packing + non-ACGT mask, per-read
+k-mer offsets, work prefix, and the read search kernel writing one node id
+(or ~0) per k-mer.
+
+Prints one JSON line. `roofline` prices the search kernel's algorithmic
+bytes (128 B per rank line touched, counted by the kernel's counted variant
+on the same batch) against the 8 TB/s HBM peak; `cpu_baseline` times the
+CPU oracle (a restatement of the reference's search: poppy rank, SB=1024,
+same presearch depth) on a bounded sample of the same k-mers, and checks
+that the GPU answers on that sample are bit-identical.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "k-mer queries/s, 1 MI355X: plain vs filtered; % of HBM random-access roof"
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--genome-bp", type=int, default=100_000_000)
+    ap.add_argument("--reads", type=int, default=1_000_000)
+    ap.add_argument("--read-len", type=int, default=150)
+    ap.add_argument("--k", type=int, default=31)
+    ap.add_argument("--presearch", type=int, default=12)
+    ap.add_argument("--layout", type=int, default=16)
+    ap.add_argument("--seg", type=int, default=256)
+    ap.add_argument("--mode", choices=["stream", "independent"], default="stream")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the index is replicated per GPU and queries shard by read set "
+              "with no exchange, so a multi-GPU figure is not produced by this build.", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import sbwt_gpu_amd as sg
+    from sbwt_gpu_amd import _capi, synth
+    from sbwt_gpu_amd.api import _ptr, _stream
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    k = args.k
+
+    # ---- index -----------------------------------------------------------
+    t = time.time()
+    g = synth.genome_codes(args.genome_bp, seed=3, gc=0.4, seg_frac=0.10, seg_len=(1_000, 50_000), seg_mut=0.01)
+    log(f"genome {g.size} bp generated in {time.time() - t:.1f}s")
+    gchars = torch.from_numpy(synth.to_ascii(g)).to(dev)
+    gstarts = torch.tensor([0, g.size], dtype=torch.int64, device=dev)
+    t = time.time()
+    ix = sg.Index.build(gchars, k, sg.Config(presearch_depth=args.presearch, layout=args.layout), starts=gstarts)
+    info = ix.info
+    log(f"index built on GPU in {time.time() - t:.1f}s: n={info['n']} columns, "
+        f"{info['device_bytes'] / 2**20:.0f} MiB, layout L{info['layout']}, m={info['presearch_depth']}")
+    del gchars, gstarts
+
+    # ---- reads -----------------------------------------------------------
+    t = time.time()
+    rchars_h, rstarts_h = synth.reads(g, args.reads, args.read_len, seed=4, sub=0.01)
+    log(f"{args.reads} reads generated in {time.time() - t:.1f}s")
+    t = time.time()
+    rchars = torch.from_numpy(rchars_h).to(dev)
+    rstarts = torch.from_numpy(rstarts_h).to(dev)
+    torch.cuda.synchronize()
+    h2d_s = time.time() - t
+    rows = args.reads
+    total = int(np.maximum(np.diff(rstarts_h) - k + 1, 0).sum())
+    L = _capi.load()
+    n_chars = rchars.numel()
+    packed = torch.empty(L.sbwt_packed_words(n_chars), dtype=torch.int64, device=dev)
+    nmask = torch.empty(L.sbwt_nmask_words(n_chars), dtype=torch.int64, device=dev)
+    offsets = torch.empty(rows + 1, dtype=torch.int64, device=dev)
+    work = torch.empty(rows + 1, dtype=torch.int64, device=dev)
+    out = torch.empty(total, dtype=torch.int64, device=dev)
+    bound = L.sbwt_reads_work_bound(rows, total, args.seg)
+    flags = _capi.READS_INDEPENDENT if args.mode == "independent" else 0
+    check = _capi.check
+    s = _stream()
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * max(args.steps, 1))]
+
+    def step(i=None, extra_flags=0):
+        check(L.sbwt_pack(_ptr(rchars), n_chars, _ptr(packed), _ptr(nmask), s))
+        check(L.sbwt_kmer_offsets(_ptr(rstarts), rows, k, _ptr(offsets), s))
+        check(L.sbwt_reads_work(_ptr(offsets), rows, args.seg, _ptr(work), s))
+        if i is not None:
+            ev[2 * i].record(stream)
+        check(L.sbwt_search_reads(ix._h, _ptr(packed), _ptr(nmask), _ptr(rstarts), _ptr(offsets), _ptr(work), rows,
+                                  bound, args.seg, _ptr(out), flags | extra_flags, s))
+        if i is not None:
+            ev[2 * i + 1].record(stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    log(f"warmup done ({args.warmup} steps), timing {args.steps} steps of {total} k-mers")
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(i)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    value = total * args.steps / elapsed
+    kern_ms = [ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(args.steps)]
+    kern_avg_s = float(np.mean(kern_ms)) / 1e3
+    log(f"{value / 1e9:.3f} G k-mers/s end to end, search kernel avg {kern_avg_s * 1e3:.3f} ms "
+        f"({total / kern_avg_s / 1e9:.3f} G k-mers/s)")
+    res_gpu = out.cpu().numpy().view(np.uint64)
+
+    # ---- algorithmic bytes: counted variant on the same batch (untimed) ----
+    ix.counters(reset=True)
+    step(None, _capi.COUNT)
+    torch.cuda.synchronize()
+    ctr = ix.counters(reset=True)
+    same = np.array_equal(out.cpu().numpy().view(np.uint64), res_gpu)
+    lines = int(ctr["lines"])
+    alg_bytes = lines * 128
+    achieved = alg_bytes / kern_avg_s / 1e9
+    found = int((res_gpu != np.uint64(0xFFFFFFFFFFFFFFFF)).sum())
+    log(f"counted run: {lines} lines ({lines / total:.2f}/k-mer), {ctr['steps']} steps, "
+        f"{ctr['full_searches']} full searches, found {found}/{total}; counted == timed: {same}")
+
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                "kernel": "k_search_reads", "kernel_avg_ms": round(kern_avg_s * 1e3, 4),
+                "lines_per_launch": lines, "bytes_per_kmer": round(alg_bytes / total, 2)}
+
+    # ---- CPU baseline + parity on a sample -----------------------------------
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(ix, rchars_h, rstarts_h, res_gpu, k, args)
+
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "k-mer queries/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic (seeded genome + reads generated in-run)",
+        "config": {"workload": f"plain GPU read search: {args.genome_bp / 1e6:g} Mbp synthetic genome (GC 40%, "
+                               f"10% segmental copies at 1%), k={k}, presearch {args.presearch}; "
+                               f"{rows} reads x {args.read_len} bp at 1% substitutions",
+                   "kmers_per_step": total, "index_columns": info["n"], "layout": f"L{info['layout']}",
+                   "mode": args.mode, "segment_kmers": args.seg,
+                   "search_kernel_kmers_per_s": round(total / kern_avg_s, 1),
+                   "found_fraction": round(found / total, 4),
+                   "h2d_reads_GBps": round(rchars_h.nbytes / h2d_s / 1e9, 2)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def cpu_baseline(ix, rchars_h, rstarts_h, res_gpu, k, args):
+    """The CPU oracle (reference search restated: poppy SB=1024, presearch m)
+    on a bounded sample of the same k-mers; also checks GPU parity there."""
+    from oracle import oracle as O
+
+    t = time.time()
+    rows = ix.rows()
+    oix = O.Index(rows[:4], ix.n, sb=1024)
+    oix.presearch(args.presearch)
+    log(f"oracle index + presearch built in {time.time() - t:.1f}s")
+    per_read = args.read_len - k + 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+
+    def sample(nreads):
+        st = rstarts_h[: nreads + 1]
+        km, _ = O.read_kmers(rchars_h[: st[-1]], st.astype(np.uint64), k)
+        return km
+
+    probe = sample(20_000)
+    t = time.perf_counter()
+    oix.search(probe, k)
+    rate = probe.size / (time.perf_counter() - t)
+    nreads = int(min(len(rstarts_h) - 1, max(20_000, rate * args.cpu_seconds / per_read)))
+    km = sample(nreads)
+    t = time.perf_counter()
+    want = oix.search(km, k)
+    dt = time.perf_counter() - t
+    parity = bool(np.array_equal(want, res_gpu[: km.size]))
+    km1 = km[: max(1, km.size // max(threads, 1) // 4)]
+    t = time.perf_counter()
+    oix.search(km1, k, threads=1)
+    rate1 = km1.size / (time.perf_counter() - t)
+    log(f"CPU oracle: {km.size / dt / 1e6:.2f} M k-mers/s on {threads} threads, {rate1 / 1e6:.2f} M/s on 1; "
+        f"GPU parity on the sample: {parity}")
+    return {"value": round(km.size / dt, 1), "unit": "k-mer queries/s", "cores": threads, "kind": "port",
+            "sample": f"first {nreads} reads ({km.size} k-mers) of the same batch, independent search, "
+                      f"poppy SB=1024, presearch {args.presearch}, OpenMP over k-mers",
+            "single_thread_value": round(rate1, 1), "gpu_parity_on_sample": parity,
+            "cpu_model": _cpu_model()}
+
+
+def _cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+if __name__ == "__main__":
+    sys.exit(main())

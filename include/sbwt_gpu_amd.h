@@ -46,6 +46,10 @@ extern "C" {
 #define SBWT_READS_INDEPENDENT 2u /* search every k-mer from the presearch table
                                      (the reference's way); default: stream
                                      consecutive k-mers through suffix groups */
+#define SBWT_COUNT 4u             /* counted variant: add the 128-byte lines the
+                                     rank reads touch to the index's counters
+                                     (sbwt_counters); also accepted by
+                                     sbwt_search_kmers_flags */
 
 typedef struct sbwt_index sbwt_index;
 
@@ -117,6 +121,20 @@ int sbwt_presearch(sbwt_index *ix, uint32_t m, void *stream);
 /* Host copies of the presearch table: 4^m intervals [left, right]. */
 int sbwt_presearch_get(const sbwt_index *ix, uint64_t *left, uint64_t *right);
 
+/* Access counters accumulated by searches run with SBWT_COUNT:
+ * out[0] = 128-byte rank lines touched (a presearch gather counts one line,
+ * a narrowing step one or two, a streaming step one or more), out[1] =
+ * narrowing + streaming steps, out[2] = k-mers answered, out[3] = k-mers
+ * that took a full search. reset != 0 zeroes them after the copy. No
+ * reference counterpart (the reference times kernels only,
+ * include/gpu/device_stream.hpp:52-78). */
+int sbwt_counters(const sbwt_index *ix, uint64_t out[4], int reset, void *stream);
+
+/* sbwt_search_kmers with flags (0 or SBWT_COUNT). */
+int sbwt_search_kmers_flags(const sbwt_index *ix, const uint64_t *d_kmers,
+                            uint64_t count, uint64_t *d_out, uint32_t flags,
+                            void *stream);
+
 /* Re-lays the rank chunks (SBWT_LAYOUT_*), e.g. for the autotuner. */
 int sbwt_index_set_layout(sbwt_index *ix, uint32_t layout, void *stream);
 
@@ -174,6 +192,27 @@ int sbwt_search_reads(const sbwt_index *ix, const uint64_t *d_packed,
                       const uint64_t *d_offsets, const uint64_t *d_work,
                       uint64_t rows, uint64_t work_bound, uint32_t seg,
                       uint64_t *d_out, uint32_t flags, void *stream);
+
+/* ---- host-side query ingest and result text ------------------------ */
+
+/* Query text -> rows: one row per line, '>' lines and empty lines skipped
+ * (KmerSearch::read_file, include/epicseq/kmer_search.hpp:319-342), a
+ * trailing '\r' dropped. chars needs len bytes, row_starts row_cap entries
+ * (rows + 1 are written). */
+int sbwt_parse_lines(const char *text, uint64_t len, uint8_t *chars,
+                     uint64_t *row_starts, uint64_t row_cap, uint64_t *rows_out);
+
+/* Result text (ResultsPersister::save_results_to_file,
+ * include/epicseq/results_persister.hpp:82-96): each answer as a signed
+ * decimal (~0 -> -1) and a space, '\n' after each row; offsets[rows+1] as
+ * from sbwt_kmer_offsets. buf NULL: only *size_out is set. */
+int sbwt_format_results(const uint64_t *results, const uint64_t *offsets,
+                        uint64_t rows, char *buf, uint64_t cap,
+                        uint64_t *size_out);
+
+/* Same text written to a file descriptor (1 = stdout). */
+int sbwt_write_results(const uint64_t *results, const uint64_t *offsets,
+                       uint64_t rows, int fd);
 
 /* Message of the calling thread's last failure ("" if none). */
 const char *sbwt_last_error(void);

@@ -13,6 +13,7 @@ SBWT_OK = 0
 ERRORS = {1: "SBWT_ERR_ARG", 2: "SBWT_ERR_HIP", 3: "SBWT_ERR_IO", 4: "SBWT_ERR_FORMAT", 5: "SBWT_ERR_UNSUPPORTED"}
 READS_N_AS_A = 1
 READS_INDEPENDENT = 2
+COUNT = 4
 
 # every function the header declares, with (restype, argtypes)
 vp, u64, u32, i32 = C.c_void_p, C.c_uint64, C.c_uint32, C.c_int
@@ -42,6 +43,8 @@ SIGNATURES = {
     "sbwt_presearch": (i32, [vp, u32, vp]),
     "sbwt_presearch_get": (i32, [vp, vp, vp]),
     "sbwt_index_set_layout": (i32, [vp, u32, vp]),
+    "sbwt_counters": (i32, [vp, u64p, i32, vp]),
+    "sbwt_search_kmers_flags": (i32, [vp, vp, u64, vp, u32, vp]),
     "sbwt_search_kmers": (i32, [vp, vp, u64, vp, vp]),
     "sbwt_search_positions": (i32, [vp, vp, vp, u64, vp]),
     "sbwt_packed_words": (u64, [u64]),
@@ -52,6 +55,9 @@ SIGNATURES = {
     "sbwt_reads_work_bound": (u64, [u64, u64, u32]),
     "sbwt_reads_work": (i32, [vp, u64, u32, vp, vp]),
     "sbwt_search_reads": (i32, [vp, vp, vp, vp, vp, vp, u64, u64, u32, vp, u32, vp]),
+    "sbwt_parse_lines": (i32, [cp, u64, vp, vp, u64, u64p]),
+    "sbwt_format_results": (i32, [vp, vp, u64, vp, u64, u64p]),
+    "sbwt_write_results": (i32, [vp, vp, u64, i32]),
     "sbwt_last_error": (cp, []),
 }
 

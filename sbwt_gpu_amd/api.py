@@ -173,12 +173,19 @@ class Index:
         check(self.lib.sbwt_index_set_layout(self._h, layout, _stream()))
 
     # ---- search ---------------------------------------------------------------
-    def search_kmers(self, kmers: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+    def search_kmers(self, kmers: torch.Tensor, out: torch.Tensor | None = None, count: bool = False) -> torch.Tensor:
         """kmers: device int64 tensor of top-aligned 2-bit k-mers."""
         if out is None:
             out = torch.empty_like(kmers)
-        check(self.lib.sbwt_search_kmers(self._h, _ptr(kmers), kmers.numel(), _ptr(out), _stream()))
+        check(self.lib.sbwt_search_kmers_flags(self._h, _ptr(kmers), kmers.numel(), _ptr(out),
+                                               _capi.COUNT if count else 0, _stream()))
         return out
+
+    def counters(self, reset: bool = True) -> dict:
+        """Access counters of searches run with count=True (see sbwt_counters)."""
+        out = (C.c_uint64 * 4)()
+        check(self.lib.sbwt_counters(self._h, out, int(reset), _stream()))
+        return {"lines": out[0], "steps": out[1], "kmers": out[2], "full_searches": out[3]}
 
     def search_positions(self, packed: torch.Tensor, positions: torch.Tensor) -> torch.Tensor:
         """In place over `positions`, like the reference kernel."""
@@ -186,12 +193,13 @@ class Index:
         return positions
 
     def search_reads(self, reads: "ReadBatch", out: torch.Tensor | None = None, n_as_a: bool = False,
-                     independent: bool = False) -> torch.Tensor:
+                     independent: bool = False, count: bool = False) -> torch.Tensor:
         if reads.k != self.k:
             raise ValueError(f"reads prepared for k={reads.k}, index has k={self.k}")
         if out is None:
             out = torch.empty(max(reads.total_kmers, 1), dtype=torch.int64, device=reads.chars.device)
-        flags = (_capi.READS_N_AS_A if n_as_a else 0) | (_capi.READS_INDEPENDENT if independent else 0)
+        flags = ((_capi.READS_N_AS_A if n_as_a else 0) | (_capi.READS_INDEPENDENT if independent else 0)
+                 | (_capi.COUNT if count else 0))
         check(self.lib.sbwt_search_reads(self._h, _ptr(reads.packed), _ptr(reads.nmask), _ptr(reads.row_starts),
                                          _ptr(reads.offsets), _ptr(reads.work), reads.rows, reads.work_bound,
                                          reads.seg, _ptr(out), flags, _stream()))
@@ -265,3 +273,32 @@ def pack(chars: torch.Tensor):
 def to_u64(t: torch.Tensor) -> np.ndarray:
     """Device int64 tensor -> host uint64 array (same bits)."""
     return t.detach().cpu().numpy().view(np.uint64)
+
+
+def parse_lines(text: bytes):
+    """Query text -> (chars uint8, row_starts int64) on the host
+    (KmerSearch::read_file, include/epicseq/kmer_search.hpp:319-342)."""
+    L = _capi.load()
+    n = len(text)
+    chars = np.zeros(max(n, 1), np.uint8)
+    cap = text.count(b"\n") + 2
+    rs = np.zeros(cap, np.uint64)
+    rows = C.c_uint64()
+    check(L.sbwt_parse_lines(text, n, C.c_void_p(chars.ctypes.data), C.c_void_p(rs.ctypes.data), cap,
+                             C.byref(rows)))
+    r = rows.value
+    return chars[: int(rs[r])], rs[: r + 1].view(np.int64)
+
+
+def format_results(results: np.ndarray, offsets: np.ndarray) -> bytes:
+    """Reference result text (include/epicseq/results_persister.hpp:82-96)."""
+    L = _capi.load()
+    res = np.ascontiguousarray(_as_u64_np(results))
+    off = np.ascontiguousarray(_as_u64_np(offsets))
+    rows = off.size - 1
+    size = C.c_uint64()
+    rp = C.c_void_p(res.ctypes.data) if res.size else None
+    check(L.sbwt_format_results(rp, C.c_void_p(off.ctypes.data), rows, None, 0, C.byref(size)))
+    buf = C.create_string_buffer(size.value + 1)
+    check(L.sbwt_format_results(rp, C.c_void_p(off.ctypes.data), rows, buf, size.value, C.byref(size)))
+    return buf.raw[: size.value]

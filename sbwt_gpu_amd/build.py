@@ -61,6 +61,15 @@ def build(jobs: int = 0, verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")
     os.replace(tmp, LIB)
+    # the command-line search over the C++ host API (include/sbwt_gpu_amd.hpp)
+    cli = os.path.join(HERE, "sbwt_search")
+    cmd = [HIPCC, "-O2", "-std=c++17", "-I", os.path.join(os.path.dirname(HERE), "include"),
+           os.path.join(HERE, "host", "sbwt_search.cpp"), "-o", cli + ".tmp", "-L", HERE, "-lsbwt_gpu_amd",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"sbwt_search build failed:\n{r.stderr[-6000:]}")
+    os.replace(cli + ".tmp", cli)
     if verbose:
         print(f"built {LIB} from {len(srcs)} sources", file=sys.stderr)
     return LIB

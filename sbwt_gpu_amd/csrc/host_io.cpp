@@ -1,0 +1,135 @@
+// Host-side query ingest and result text of libsbwt_gpu_amd.
+//
+// sbwt_parse_lines replaces KmerSearch::read_file
+// (include/epicseq/kmer_search.hpp:319-342): one row per line, lines starting
+// with '>' skipped, empty lines skipped. Two behaviours differ on purpose: a
+// trailing '\r' is dropped (the reference keeps it as a character), and a
+// last line without '\n' keeps its last character (the reference drops it,
+// because getline's count then includes no delimiter).
+//
+// sbwt_format_results / sbwt_write_results replace
+// ResultsPersister::save_results_to_file (include/epicseq/results_persister.hpp:82-96):
+// every answer as a signed decimal (~0 prints as -1) followed by a space, one
+// line per row. Rows are formatted in parallel chunks and written in order.
+#include <unistd.h>
+
+#include <algorithm>
+#include <charconv>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/sbwt_gpu_amd.h"
+
+namespace sbwt {
+int fail(int code, const char* fmt, ...);
+void clear_error();
+}  // namespace sbwt
+
+namespace {
+
+using u64 = uint64_t;
+
+// Formats rows [r0, r1) into out.
+void format_rows(const u64* res, const u64* off, u64 r0, u64 r1, std::string& out) {
+  char tmp[24];
+  for (u64 r = r0; r < r1; r++) {
+    for (u64 j = off[r]; j < off[r + 1]; j++) {
+      auto p = std::to_chars(tmp, tmp + sizeof tmp, static_cast<long long>(static_cast<int64_t>(res[j])));
+      out.append(tmp, p.ptr);
+      out.push_back(' ');
+    }
+    out.push_back('\n');
+  }
+}
+
+// Splits rows into chunks of roughly equal k-mer counts.
+std::vector<u64> chunk_rows(const u64* off, u64 rows, unsigned parts) {
+  std::vector<u64> cut{0};
+  const u64 total = off[rows] - off[0];
+  for (unsigned p = 1; p < parts; p++) {
+    const u64 target = off[0] + total * p / parts;
+    const u64 r = static_cast<u64>(std::lower_bound(off, off + rows, target) - off);
+    if (r > cut.back() && r < rows) cut.push_back(r);
+  }
+  cut.push_back(rows);
+  return cut;
+}
+
+std::vector<std::string> format_all(const u64* res, const u64* off, u64 rows) {
+  unsigned hw = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+  if (off[rows] - off[0] < (1u << 16)) hw = 1;
+  const auto cut = chunk_rows(off, rows, hw);
+  std::vector<std::string> parts(cut.size() - 1);
+  std::vector<std::thread> th;
+  for (size_t i = 0; i + 1 < cut.size(); i++)
+    th.emplace_back([&, i] { format_rows(res, off, cut[i], cut[i + 1], parts[i]); });
+  for (auto& t : th) t.join();
+  return parts;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sbwt_parse_lines(const char* text, uint64_t len, uint8_t* chars, uint64_t* row_starts, uint64_t row_cap,
+                     uint64_t* rows_out) {
+  sbwt::clear_error();
+  if ((!text && len) || !chars || !row_starts || !rows_out)
+    return sbwt::fail(SBWT_ERR_ARG, "sbwt_parse_lines: NULL argument");
+  u64 rows = 0, nch = 0, p = 0;
+  while (p < len) {
+    const char* nl = static_cast<const char*>(memchr(text + p, '\n', len - p));
+    const u64 e = nl ? static_cast<u64>(nl - text) : len;
+    u64 le = e;
+    if (le > p && text[le - 1] == '\r') le--;
+    if (le > p && text[p] != '>') {
+      if (rows + 1 >= row_cap) return sbwt::fail(SBWT_ERR_ARG, "sbwt_parse_lines: more than %llu rows",
+                                                 static_cast<unsigned long long>(row_cap - 1));
+      row_starts[rows++] = nch;
+      memcpy(chars + nch, text + p, le - p);
+      nch += le - p;
+    }
+    p = e + 1;
+  }
+  row_starts[rows] = nch;
+  *rows_out = rows;
+  return SBWT_OK;
+}
+
+int sbwt_format_results(const uint64_t* results, const uint64_t* offsets, uint64_t rows, char* buf, uint64_t cap,
+                        uint64_t* size_out) {
+  sbwt::clear_error();
+  if (!offsets || !size_out || (rows && !results)) return sbwt::fail(SBWT_ERR_ARG, "sbwt_format_results: NULL argument");
+  const auto parts = format_all(results, offsets, rows);
+  u64 size = 0;
+  for (const auto& s : parts) size += s.size();
+  *size_out = size;
+  if (!buf) return SBWT_OK;
+  if (cap < size) return sbwt::fail(SBWT_ERR_ARG, "sbwt_format_results: buffer of %llu bytes, %llu needed",
+                                    static_cast<unsigned long long>(cap), static_cast<unsigned long long>(size));
+  u64 o = 0;
+  for (const auto& s : parts) {
+    memcpy(buf + o, s.data(), s.size());
+    o += s.size();
+  }
+  return SBWT_OK;
+}
+
+int sbwt_write_results(const uint64_t* results, const uint64_t* offsets, uint64_t rows, int fd) {
+  sbwt::clear_error();
+  if (!offsets || (rows && !results)) return sbwt::fail(SBWT_ERR_ARG, "sbwt_write_results: NULL argument");
+  const auto parts = format_all(results, offsets, rows);
+  for (const auto& s : parts) {
+    size_t o = 0;
+    while (o < s.size()) {
+      const ssize_t w = ::write(fd, s.data() + o, s.size() - o);
+      if (w <= 0) return sbwt::fail(SBWT_ERR_IO, "writing the results failed");
+      o += static_cast<size_t>(w);
+    }
+  }
+  return SBWT_OK;
+}
+
+}  // extern "C"

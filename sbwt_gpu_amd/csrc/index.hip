@@ -49,6 +49,7 @@ void destroy_index(sbwt_index* ix) {
   if (ix->d_rows) (void)hipFree(ix->d_rows);
   if (ix->d_lines) (void)hipFree(ix->d_lines);
   if (ix->d_pre) (void)hipFree(ix->d_pre);
+  if (ix->d_counters) (void)hipFree(ix->d_counters);
   delete ix;
 }
 
@@ -108,12 +109,14 @@ int build_layout(sbwt_index* ix, hipStream_t s) {
   const u32 pw = cb / 4 - 1;
   const u32 cols = pw * 32;
   const u64 nchunks = chunks_for(ix->n, cols);
-  const u64 row_q = nchunks * (cb / 16);
+  // rows start on 128-byte lines, so chunk ch of any row lies in line ch*cb/128
+  const u64 row_q = ((nchunks * (cb / 16) + 7) / 8) * 8;
   if (ix->d_lines) {
     SBWT_HIP(hipFree(ix->d_lines));
     ix->d_lines = nullptr;
   }
   SBWT_HIP(hipMalloc(&ix->d_lines, kRows * row_q * sizeof(uint4)));
+  SBWT_HIP(hipMemsetAsync(ix->d_lines, 0, kRows * row_q * sizeof(uint4), s));
   ix->row_q = row_q;
 
   u32 *d_counts = nullptr, *d_prefix = nullptr;
@@ -212,6 +215,8 @@ __global__ void k_clear_tail(u64* rows, u64 nwords, u64 n, u32 mark_root) {
 }
 
 int finish_index(sbwt_index* ix, hipStream_t s) {
+  SBWT_HIP(hipMalloc(&ix->d_counters, kCounters * sizeof(unsigned long long)));
+  SBWT_HIP(hipMemsetAsync(ix->d_counters, 0, kCounters * sizeof(unsigned long long), s));
   {
     dim3 grid(static_cast<u32>((ix->nwords + 255) / 256), kRows);
     k_clear_tail<<<grid, 256, 0, s>>>(ix->d_rows, ix->nwords, ix->n, ix->has_groups);
@@ -476,6 +481,16 @@ int sbwt_presearch_get(const sbwt_index* ix, uint64_t* left, uint64_t* right) {
     left[i] = t[i].x;
     right[i] = t[i].y;
   }
+  return SBWT_OK;
+}
+
+int sbwt_counters(const sbwt_index* ix, uint64_t out[4], int reset, void* stream) {
+  clear_error();
+  SBWT_REQUIRE(ix && out, "index and out must not be NULL");
+  hipStream_t s = as_stream(stream);
+  SBWT_HIP(hipMemcpyAsync(out, ix->d_counters, kCounters * sizeof(u64), hipMemcpyDeviceToHost, s));
+  if (reset) SBWT_HIP(hipMemsetAsync(ix->d_counters, 0, kCounters * sizeof(u64), s));
+  SBWT_HIP(hipStreamSynchronize(s));
   return SBWT_OK;
 }
 

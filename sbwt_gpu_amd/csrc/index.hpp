@@ -19,6 +19,8 @@ struct sbwt_index {
   // presearch table of 4^m intervals
   sbwt::u32 m = 0;
   uint2* d_pre = nullptr;
+  // access counters of the counted search variants (lines, steps, k-mers, full searches)
+  unsigned long long* d_counters = nullptr;
   sbwt_config cfg{};
   sbwt::u64 num_kmers = 0;  // |K| when built on the GPU, else 0
 };
@@ -28,6 +30,7 @@ namespace sbwt {
 constexpr u32 kRows = 5;    // A, C, G, T, suffix-group starts
 constexpr u32 kGroupRow = 4;
 constexpr u32 kMaxPresearch = 14;
+constexpr u32 kCounters = 4;
 
 inline u64 words_for(u64 n) { return ((n + 4095) / 4096) * 64; }
 

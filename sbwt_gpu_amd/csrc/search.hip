@@ -13,22 +13,38 @@
 
 namespace sbwt {
 
-// Full search of a top-aligned k-mer (char 0 in bits 63:62).
+// Per-lane access tally of the counted variants (compiled out otherwise).
+struct Tally {
+  u32 lines = 0, steps = 0, full = 0;
+};
+
 template <int CB>
-__device__ __forceinline__ u64 full_search(const DevIndex& ix, u64 kmer) {
+__device__ __forceinline__ u32 line_of(u32 col) {
+  return static_cast<u32>((static_cast<u64>(col / Layout<CB>::COLS) * CB) >> 7);
+}
+
+// Full search of a top-aligned k-mer (char 0 in bits 63:62).
+template <int CB, bool COUNT>
+__device__ __forceinline__ u64 full_search(const DevIndex& ix, u64 kmer, Tally& t) {
   u32 l, r, i;
   if (ix.m) {
     const uint2 e = ix.pre[kmer >> (64 - 2 * ix.m)];
     l = e.x;
     r = e.y;
     i = ix.m;
+    if (COUNT) t.lines++;
   } else {
     l = 0;
     r = ix.C[4] - 1u;
     i = 0;
   }
+  if (COUNT) t.full++;
   for (; i < ix.k; i++) {
     const u32 c = static_cast<u32>(kmer >> (62 - 2 * i)) & 3u;
+    if (COUNT) {
+      t.lines += line_of<CB>(l) == line_of<CB>(r + 1u) ? 1u : 2u;
+      t.steps++;
+    }
     narrow<CB>(ix, c, l, r);
     if (l > r) break;
   }
@@ -36,32 +52,52 @@ __device__ __forceinline__ u64 full_search(const DevIndex& ix, u64 kmer) {
 }
 
 // Next k-mer after a found one at node j, appending character c.
-template <int CB>
-__device__ __forceinline__ u64 stream_step(const DevIndex& ix, u32 j, u32 c) {
+template <int CB, bool COUNT>
+__device__ __forceinline__ u64 stream_step(const DevIndex& ix, u32 j, u32 c, Tally& t) {
   constexpr u32 COLS = Layout<CB>::COLS;
   const uint4* row = row_of(ix, c);
   u32 ch = j / COLS;
   u32 o = j - ch * COLS;
   Chunk<CB> q;
   load_chunk<CB>(row, ch, q);
+  if (COUNT) {
+    t.lines++;
+    t.steps++;
+  }
   // a set bit means j starts its group and has the c-edge
   if (bit_chunk<CB>(q, o)) return static_cast<u64>(c_of(ix, c) + rank_chunk<CB>(q, o));
   // otherwise find the group start g <= j; column 0 always starts a group
   const uint4* grow = row_of(ix, kGroupRow);
   Chunk<CB> gq;
   load_chunk<CB>(grow, ch, gq);
+  if (COUNT) t.lines++;
   int p = last_one_upto<CB>(gq, o);
   while (p < 0 && ch > 0) {
     ch--;
     load_chunk<CB>(grow, ch, gq);
+    if (COUNT && line_of<CB>(ch * COLS) != line_of<CB>((ch + 1) * COLS)) t.lines++;
     p = last_one_upto<CB>(gq, COLS - 1);
   }
   const u32 g = p < 0 ? 0u : ch * COLS + static_cast<u32>(p);
   if (g == j) return kNone;
   const u32 gch = g / COLS, go = g - gch * COLS;
   load_chunk<CB>(row, gch, q);
+  if (COUNT && line_of<CB>(g) != line_of<CB>(j)) t.lines++;
   if (!bit_chunk<CB>(q, go)) return kNone;
   return static_cast<u64>(c_of(ix, c) + rank_chunk<CB>(q, go));
+}
+
+// Adds a lane's tally to the index counters (one atomic per counter per wave).
+template <bool COUNT>
+__device__ __forceinline__ void flush(unsigned long long* ctr, const Tally& t, u32 kmers) {
+  if (!COUNT) return;
+  u64 v[4] = {t.lines, t.steps, kmers, t.full};
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    u64 x = v[i];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(ctr + i, static_cast<unsigned long long>(x));
+  }
 }
 
 // The reference's two-word extraction (search_kernel.hpp:66-68): the 64
@@ -76,12 +112,13 @@ __device__ __forceinline__ u64 extract1(const u64* __restrict__ w, u64 pos) {
   return (w[pos >> 6] << (pos & 63)) | ((w[(pos >> 6) + 1] >> (63 - (pos & 63))) >> 1);
 }
 
-template <int CB>
+template <int CB, bool COUNT>
 __global__ __launch_bounds__(256) void k_search_kmers(DevIndex ix, const u64* __restrict__ kmers, u64 count,
-                                                      u64* __restrict__ out) {
+                                                      u64* __restrict__ out, unsigned long long* ctr) {
   const u64 q = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x;
-  if (q >= count) return;
-  out[q] = full_search<CB>(ix, kmers[q]);
+  Tally t;
+  if (q < count) out[q] = full_search<CB, COUNT>(ix, kmers[q], t);
+  flush<COUNT>(ctr, t, q < count ? 1u : 0u);
 }
 
 template <int CB>
@@ -89,56 +126,66 @@ __global__ __launch_bounds__(256) void k_search_positions(DevIndex ix, const u64
                                                           u64* __restrict__ pos_res, u64 count) {
   const u64 q = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x;
   if (q >= count) return;
-  pos_res[q] = full_search<CB>(ix, extract2(packed, pos_res[q]));
+  Tally t;
+  pos_res[q] = full_search<CB, false>(ix, extract2(packed, pos_res[q]), t);
 }
 
 // One lane per work item: up to `seg` consecutive k-mers of one row.
-template <int CB, bool STREAM, bool NBREAK>
+template <int CB, bool STREAM, bool NBREAK, bool COUNT>
 __global__ __launch_bounds__(256) void k_search_reads(DevIndex ix, const u64* __restrict__ packed,
                                                       const u64* __restrict__ nmask,
                                                       const u64* __restrict__ row_starts,
                                                       const u64* __restrict__ offsets,
                                                       const u64* __restrict__ work, u64 rows, u64 bound, u32 seg,
-                                                      u64* __restrict__ out) {
+                                                      u64* __restrict__ out, unsigned long long* ctr) {
   const u64 gid = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x;
-  if (gid >= bound || rows == 0) return;
-  if (gid >= work[rows]) return;
-  // the row: work[r] <= gid < work[r+1]
-  u64 lo = 0, hi = rows;
-  while (hi - lo > 1) {
-    const u64 mid = (lo + hi) >> 1;
-    if (work[mid] <= gid) lo = mid; else hi = mid;
-  }
-  const u64 r = lo;
-  const u64 nk = offsets[r + 1] - offsets[r];
-  const u64 j0 = (gid - work[r]) * seg;
-  const u64 cnt = (nk - j0) < seg ? (nk - j0) : seg;
-  const u64 c0 = row_starts[r] + j0;
-  u64* o = out + offsets[r] + j0;
-  const u32 k = ix.k;
-  bool prev = false;
-  u32 pnode = 0;
-  for (u64 j = 0; j < cnt; j++) {
-    const u64 pos = c0 + j;
-    const u64 kmer = extract2(packed, pos);
-    u64 res;
-    if (NBREAK && (extract1(nmask, pos) >> (64 - k)) != 0) {
-      res = kNone;
-    } else if (STREAM && prev) {
-      res = stream_step<CB>(ix, pnode, static_cast<u32>(kmer >> (64 - 2 * k)) & 3u);
-    } else {
-      res = full_search<CB>(ix, kmer);
+  Tally t;
+  u32 done = 0;
+  if (gid < bound && rows != 0 && gid < work[rows]) {
+    // the row: work[r] <= gid < work[r+1]
+    u64 lo = 0, hi = rows;
+    while (hi - lo > 1) {
+      const u64 mid = (lo + hi) >> 1;
+      if (work[mid] <= gid) lo = mid; else hi = mid;
     }
-    prev = res != kNone;
-    pnode = static_cast<u32>(res);
-    o[j] = res;
+    const u64 r = lo;
+    const u64 nk = offsets[r + 1] - offsets[r];
+    const u64 j0 = (gid - work[r]) * seg;
+    const u64 cnt = (nk - j0) < seg ? (nk - j0) : seg;
+    const u64 c0 = row_starts[r] + j0;
+    u64* o = out + offsets[r] + j0;
+    const u32 k = ix.k;
+    bool prev = false;
+    u32 pnode = 0;
+    for (u64 j = 0; j < cnt; j++) {
+      const u64 pos = c0 + j;
+      const u64 kmer = extract2(packed, pos);
+      u64 res;
+      if (NBREAK && (extract1(nmask, pos) >> (64 - k)) != 0) {
+        res = kNone;
+      } else if (STREAM && prev) {
+        res = stream_step<CB, COUNT>(ix, pnode, static_cast<u32>(kmer >> (64 - 2 * k)) & 3u, t);
+      } else {
+        res = full_search<CB, COUNT>(ix, kmer, t);
+      }
+      prev = res != kNone;
+      pnode = static_cast<u32>(res);
+      o[j] = res;
+    }
+    done = static_cast<u32>(cnt);
   }
+  flush<COUNT>(ctr, t, done);
 }
 
 template <int CB>
-static int launch_kmers(const sbwt_index* ix, const u64* kmers, u64 count, u64* out, hipStream_t s) {
+static int launch_kmers(const sbwt_index* ix, const u64* kmers, u64 count, u64* out, bool count_lines,
+                        hipStream_t s) {
   const u32 bt = 256;
-  k_search_kmers<CB><<<(count + bt - 1) / bt, bt, 0, s>>>(dev_view(ix), kmers, count, out);
+  const u64 grid = (count + bt - 1) / bt;
+  if (count_lines)
+    k_search_kmers<CB, true><<<grid, bt, 0, s>>>(dev_view(ix), kmers, count, out, ix->d_counters);
+  else
+    k_search_kmers<CB, false><<<grid, bt, 0, s>>>(dev_view(ix), kmers, count, out, ix->d_counters);
   SBWT_HIP(hipGetLastError());
   return SBWT_OK;
 }
@@ -151,26 +198,35 @@ static int launch_positions(const sbwt_index* ix, const u64* packed, u64* pos, u
   return SBWT_OK;
 }
 
-template <int CB, bool STREAM, bool NBREAK>
-static int launch_reads3(const sbwt_index* ix, const u64* packed, const u64* nmask, const u64* rs, const u64* off,
-                         const u64* work, u64 rows, u64 bound, u32 seg, u64* out, hipStream_t s) {
+struct ReadArgs {
+  const u64 *packed, *nmask, *rs, *off, *work;
+  u64 rows, bound;
+  u32 seg;
+  u64* out;
+};
+
+template <int CB, bool STREAM, bool NBREAK, bool COUNT>
+static int launch_reads4(const sbwt_index* ix, const ReadArgs& a, hipStream_t s) {
   const u32 bt = 256;
-  k_search_reads<CB, STREAM, NBREAK>
-      <<<(bound + bt - 1) / bt, bt, 0, s>>>(dev_view(ix), packed, nmask, rs, off, work, rows, bound, seg, out);
+  k_search_reads<CB, STREAM, NBREAK, COUNT><<<(a.bound + bt - 1) / bt, bt, 0, s>>>(
+      dev_view(ix), a.packed, a.nmask, a.rs, a.off, a.work, a.rows, a.bound, a.seg, a.out, ix->d_counters);
   SBWT_HIP(hipGetLastError());
   return SBWT_OK;
 }
 
-template <int CB>
-static int launch_reads(const sbwt_index* ix, bool stream, bool nbreak, const u64* packed, const u64* nmask,
-                        const u64* rs, const u64* off, const u64* work, u64 rows, u64 bound, u32 seg, u64* out,
-                        hipStream_t s) {
-  if (stream && nbreak) return launch_reads3<CB, true, true>(ix, packed, nmask, rs, off, work, rows, bound, seg, out, s);
-  if (stream) return launch_reads3<CB, true, false>(ix, packed, nmask, rs, off, work, rows, bound, seg, out, s);
-  if (nbreak) return launch_reads3<CB, false, true>(ix, packed, nmask, rs, off, work, rows, bound, seg, out, s);
-  return launch_reads3<CB, false, false>(ix, packed, nmask, rs, off, work, rows, bound, seg, out, s);
+template <int CB, bool STREAM, bool NBREAK>
+static int launch_reads3(const sbwt_index* ix, const ReadArgs& a, bool count, hipStream_t s) {
+  return count ? launch_reads4<CB, STREAM, NBREAK, true>(ix, a, s) : launch_reads4<CB, STREAM, NBREAK, false>(ix, a, s);
 }
 
+template <int CB>
+static int launch_reads(const sbwt_index* ix, const ReadArgs& a, bool stream, bool nbreak, bool count,
+                        hipStream_t s) {
+  if (stream && nbreak) return launch_reads3<CB, true, true>(ix, a, count, s);
+  if (stream) return launch_reads3<CB, true, false>(ix, a, count, s);
+  if (nbreak) return launch_reads3<CB, false, true>(ix, a, count, s);
+  return launch_reads3<CB, false, false>(ix, a, count, s);
+}
 
 }  // namespace sbwt
 
@@ -178,18 +234,24 @@ using namespace sbwt;
 
 extern "C" {
 
-int sbwt_search_kmers(const sbwt_index* ix, const uint64_t* d_kmers, uint64_t count, uint64_t* d_out,
-                      void* stream) {
+int sbwt_search_kmers_flags(const sbwt_index* ix, const uint64_t* d_kmers, uint64_t count, uint64_t* d_out,
+                            uint32_t flags, void* stream) {
   clear_error();
   SBWT_REQUIRE(ix, "index is NULL");
   if (count == 0) return SBWT_OK;
   SBWT_REQUIRE(d_kmers && d_out, "device buffers must not be NULL");
+  const bool cnt = (flags & SBWT_COUNT) != 0;
   hipStream_t s = as_stream(stream);
   switch (ix->layout) {
-    case 16: return launch_kmers<16>(ix, d_kmers, count, d_out, s);
-    case 32: return launch_kmers<32>(ix, d_kmers, count, d_out, s);
-    default: return launch_kmers<64>(ix, d_kmers, count, d_out, s);
+    case 16: return launch_kmers<16>(ix, d_kmers, count, d_out, cnt, s);
+    case 32: return launch_kmers<32>(ix, d_kmers, count, d_out, cnt, s);
+    default: return launch_kmers<64>(ix, d_kmers, count, d_out, cnt, s);
   }
+}
+
+int sbwt_search_kmers(const sbwt_index* ix, const uint64_t* d_kmers, uint64_t count, uint64_t* d_out,
+                      void* stream) {
+  return sbwt_search_kmers_flags(ix, d_kmers, count, d_out, 0, stream);
 }
 
 int sbwt_search_positions(const sbwt_index* ix, const uint64_t* d_packed, uint64_t* d_positions, uint64_t count,
@@ -221,17 +283,13 @@ int sbwt_search_reads(const sbwt_index* ix, const uint64_t* d_packed, const uint
                 "streaming search needs the suffix-group row; open the index with it or pass "
                 "SBWT_READS_INDEPENDENT");
   if (seg == 0) seg = 256;
+  const bool cnt = (flags & SBWT_COUNT) != 0;
+  const ReadArgs a{d_packed, d_nmask, d_row_starts, d_offsets, d_work, rows, work_bound, seg, d_out};
   hipStream_t s = as_stream(stream);
   switch (ix->layout) {
-    case 16:
-      return launch_reads<16>(ix, stream_mode, nbreak, d_packed, d_nmask, d_row_starts, d_offsets, d_work, rows,
-                              work_bound, seg, d_out, s);
-    case 32:
-      return launch_reads<32>(ix, stream_mode, nbreak, d_packed, d_nmask, d_row_starts, d_offsets, d_work, rows,
-                              work_bound, seg, d_out, s);
-    default:
-      return launch_reads<64>(ix, stream_mode, nbreak, d_packed, d_nmask, d_row_starts, d_offsets, d_work, rows,
-                              work_bound, seg, d_out, s);
+    case 16: return launch_reads<16>(ix, a, stream_mode, nbreak, cnt, s);
+    case 32: return launch_reads<32>(ix, a, stream_mode, nbreak, cnt, s);
+    default: return launch_reads<64>(ix, a, stream_mode, nbreak, cnt, s);
   }
 }
 

@@ -1,0 +1,103 @@
+"""Seeded synthetic genomes and reads for the benchmark workloads.
+
+All draws come from numpy's PCG64 with fixed seeds, so every run builds the
+same index and queries the same reads.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+ACGT = np.frombuffer(b"ACGT", dtype=np.uint8)
+
+
+def genome_codes(length: int, seed: int, gc: float = 0.5, seg_frac: float = 0.0,
+                 seg_len=(1_000, 50_000), seg_mut: float = 0.01) -> np.ndarray:
+    """2-bit codes (0..3 = A,C,G,T) of a genome with the given GC fraction;
+    seg_frac of it is then overwritten by copies of random source segments
+    (lengths uniform in seg_len) carrying i.i.d. substitutions at seg_mut."""
+    rng = np.random.default_rng(seed)
+    at = (1.0 - gc) / 2
+    codes = np.empty(length, dtype=np.uint8)
+    chunk = 1 << 24
+    for s in range(0, length, chunk):
+        e = min(length, s + chunk)
+        codes[s:e] = rng.choice(4, size=e - s, p=[at, gc / 2, gc / 2, at])
+    covered, target = 0, int(seg_frac * length)
+    while covered < target:
+        L = int(rng.integers(seg_len[0], seg_len[1] + 1))
+        L = min(L, length // 2)
+        src = int(rng.integers(0, length - L))
+        dst = int(rng.integers(0, length - L))
+        seg = codes[src:src + L].copy()
+        flip = np.flatnonzero(rng.random(L) < seg_mut)
+        seg[flip] = (seg[flip] + rng.integers(1, 4, flip.size)) % 4
+        codes[dst:dst + L] = seg
+        covered += L
+    return codes
+
+
+def to_ascii(codes: np.ndarray) -> np.ndarray:
+    return ACGT[codes]
+
+
+def reads(genome_codes_: np.ndarray, count: int, length: int, seed: int, sub: float = 0.01,
+          n_rate: float = 0.0):
+    """`count` reads of `length` from uniform positions, each base substituted
+    (by one of the three other bases) with probability `sub`, and set to N
+    with probability `n_rate`. Returns (ASCII uint8 array of the concatenated
+    reads, row starts int64[count+1])."""
+    rng = np.random.default_rng(seed)
+    G = genome_codes_.size
+    out = np.empty(count * length, dtype=np.uint8)
+    step = max(1, (1 << 24) // length)
+    ar = np.arange(length, dtype=np.int64)
+    for s in range(0, count, step):
+        e = min(count, s + step)
+        pos = rng.integers(0, G - length + 1, e - s)
+        c = genome_codes_[pos[:, None] + ar[None, :]]
+        flip = rng.random(c.shape) < sub
+        c[flip] = (c[flip] + rng.integers(1, 4, int(flip.sum()))) % 4
+        a = ACGT[c]
+        if n_rate > 0:
+            a[rng.random(a.shape) < n_rate] = ord("N")
+        out[s * length:e * length] = a.reshape(-1)
+    starts = np.arange(count + 1, dtype=np.int64) * length
+    return out, starts
+
+
+def lognormal_read_sets(genome_codes_: np.ndarray, sets: int, seed: int, max_reads: int = 5000,
+                        median: float = 300.0, sigma: float = 0.8, lo: int = 100, hi: int = 20_000,
+                        sub: float = 0.01, n_rate: float = 0.05, max_chars: int | None = None):
+    """Read sets of uneven size: reads per set ~ U{1..max_reads}, read length
+    ~ lognormal(ln median, sigma) clipped to [lo, hi], substitutions at `sub`,
+    N at `n_rate`. Returns (ASCII chars, row starts, set starts over rows)."""
+    rng = np.random.default_rng(seed)
+    per_set = rng.integers(1, max_reads + 1, sets)
+    nrows = int(per_set.sum())
+    lens = np.clip(np.exp(rng.normal(np.log(median), sigma, nrows)).astype(np.int64), lo, hi)
+    if max_chars is not None and lens.sum() > max_chars:
+        keep = int(np.searchsorted(np.cumsum(lens), max_chars))
+        lens = lens[:keep]
+        cs = np.cumsum(per_set)
+        per_set = np.diff(np.concatenate([[0], np.minimum(cs, keep)]))
+        per_set = per_set[per_set > 0]
+        nrows = keep
+    starts = np.zeros(nrows + 1, dtype=np.int64)
+    starts[1:] = np.cumsum(lens)
+    out = np.empty(int(starts[-1]), dtype=np.uint8)
+    G = genome_codes_.size
+    for r in range(0, nrows, 4096):
+        e = min(nrows, r + 4096)
+        for i in range(r, e):
+            L = int(lens[i])
+            p = int(rng.integers(0, G - L + 1))
+            out[starts[i]:starts[i + 1]] = genome_codes_[p:p + L]
+    codes = out
+    flip = rng.random(codes.size) < sub
+    codes[flip] = (codes[flip] + rng.integers(1, 4, int(flip.sum()))) % 4
+    chars = ACGT[codes]
+    if n_rate > 0:
+        chars[rng.random(chars.size) < n_rate] = ord("N")
+    set_starts = np.zeros(per_set.size + 1, dtype=np.int64)
+    set_starts[1:] = np.cumsum(per_set)
+    return chars, starts, set_starts

@@ -1,0 +1,55 @@
+"""The command-line search (C++ host API over the C ABI) against the oracle:
+bit-vector files in, reference result text out."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():  # pragma: no cover
+    pytest.skip("needs a GPU", allow_module_level=True)
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "sbwt_gpu_amd", "sbwt_search")
+ACGT = np.array(list("ACGT"))
+
+
+@pytest.mark.parametrize("extra", [[], ["--independent"], ["--without-presearch"], ["--n-as-a"], ["--layout=32"]])
+def test_cli_matches_oracle_text(tmp_path, extra):
+    rng = np.random.default_rng(8)
+    g = "".join(ACGT[rng.integers(0, 4, 50_000)])
+    k = 27
+    s = O.SBWT([g], k)
+    paths = [str(tmp_path / f"{c}.bin") for c in "ACGT"]
+    for c in range(4):
+        O.bv_write(paths[c], s.rows[c], s.n)
+    O.bv_write(str(tmp_path / "grp.bin"), s.group, s.n)
+    lines = [">header"]
+    for i in range(300):
+        p = int(rng.integers(0, len(g) - 200))
+        r = np.array(list(g[p:p + int(rng.integers(10, 200))]))
+        r[rng.random(r.size) < 0.01] = "N"
+        lines.append("".join(r) if i % 5 else "".join(ACGT[rng.integers(0, 4, 80)]))
+    text = ("\n".join(lines) + "\n").encode()
+    q = tmp_path / "q.txt"
+    q.write_bytes(text)
+    args = [CLI, f"--k={k}", "--k-presearch=9", *[f"--file-{c}={p}" for c, p in zip("acgt", paths)],
+            f"--file-groups={tmp_path / 'grp.bin'}", f"--file-queries={q}", *extra]
+    out = subprocess.run(args, capture_output=True, timeout=120)
+    assert out.returncode == 0, out.stderr.decode()
+    oix = O.Index(s.rows, s.n)
+    oix.presearch(0 if "--without-presearch" in extra else 9)
+    chars, rs = O.parse_lines(text)
+    want = O.search_reads(oix, chars, rs, k, n_breaks="--n-as-a" not in extra)
+    assert out.stdout == O.format_results(want, O.kmer_offsets(rs, k))
+
+
+def test_cli_errors(tmp_path):
+    r = subprocess.run([CLI, "--k=31", f"--file-a={tmp_path}/nope", f"--file-c={tmp_path}/nope",
+                        f"--file-g={tmp_path}/nope", f"--file-t={tmp_path}/nope", "--file-queries=x"],
+                       capture_output=True, timeout=60)
+    assert r.returncode == 1 and b"can not be opened" in r.stderr
