@@ -52,10 +52,20 @@ def main() -> int:
     ap.add_argument("--presearch", type=int, default=12)
     ap.add_argument("--layout", type=int, default=16)
     ap.add_argument("--seg", type=int, default=256)
-    ap.add_argument("--mode", choices=["stream", "independent"], default="stream")
+    ap.add_argument("--mode", choices=["default", "nocert", "persistent", "persistent-nocert", "independent"],
+                    default="default",
+                    help="read search variant: one lane per read segment with streaming + certificates "
+                         "(default), without certificates, the persistent lane-refill kernel (with or without "
+                         "certificates), or every k-mer searched independently as the reference does")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-roof", action="store_true", help="skip the random-gather roof measurement")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy 3 times (HBM counter calibration) and exit")
     args = ap.parse_args()
+
+    if args.calib_copy:
+        return calib_copy()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the index is replicated per GPU and queries shard by read set "
@@ -104,7 +114,9 @@ def main() -> int:
     work = torch.empty(rows + 1, dtype=torch.int64, device=dev)
     out = torch.empty(total, dtype=torch.int64, device=dev)
     bound = L.sbwt_reads_work_bound(rows, total, args.seg)
-    flags = _capi.READS_INDEPENDENT if args.mode == "independent" else 0
+    flags = {"default": 0, "nocert": _capi.READS_NO_CERT, "persistent": _capi.READS_PERSISTENT,
+             "persistent-nocert": _capi.READS_PERSISTENT | _capi.READS_NO_CERT,
+             "independent": _capi.READS_INDEPENDENT}[args.mode]
     check = _capi.check
     s = _stream()
     stream = torch.cuda.current_stream()
@@ -153,8 +165,13 @@ def main() -> int:
 
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-                "kernel": "k_search_reads", "kernel_avg_ms": round(kern_avg_s * 1e3, 4),
-                "lines_per_launch": lines, "bytes_per_kmer": round(alg_bytes / total, 2)}
+                "kernel": "k_reads_v2" if args.mode.startswith("persistent") else "k_search_reads", "kernel_avg_ms": round(kern_avg_s * 1e3, 4),
+                "lines_per_launch": lines, "bytes_per_kmer": round(alg_bytes / total, 2),
+                "index_bytes": info["device_bytes"]}
+    if not args.no_roof:
+        roof = gather_roof()
+        roofline["measured_random_roof"] = roof
+        roofline["frac_of_measured_roof"] = round(achieved / roof["GBps_128B_lines"], 4)
 
     # ---- CPU baseline + parity on a sample -----------------------------------
     cpu = None
@@ -178,6 +195,59 @@ def main() -> int:
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    return 0
+
+
+def gather_roof(gib: int = 4):
+    """Uniform-random 128-byte-line gathers over a `gib` GiB buffer (far
+    beyond the 256 MiB last-level cache): the random-access roof. Two shapes:
+    one 16-byte read per random line (the search's access), and whole lines
+    read by 8 lanes together."""
+    import torch
+
+    from sbwt_gpu_amd import _capi
+    from sbwt_gpu_amd.api import _ptr, _stream
+
+    L = _capi.load()
+    nbytes = gib << 30
+    buf = torch.empty(nbytes // 8, dtype=torch.int64, device="cuda")
+    buf.fill_(0x5A5A)
+    lanes, per = 256 * 1024 * 8, 64
+    sink = torch.empty(lanes, dtype=torch.int64, device="cuda")
+    s = _stream()
+    st = torch.cuda.current_stream()
+    out = {"buffer_GiB": gib}
+    for name, full in (("one_16B_read_per_line", 0), ("whole_line_8_lanes", 1)):
+        _capi.check(L.sbwt_gather_roof(_ptr(buf), nbytes, 128, full, per, lanes, 7, _ptr(sink), s))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 5
+        e0.record(st)
+        for r in range(reps):
+            _capi.check(L.sbwt_gather_roof(_ptr(buf), nbytes, 128, full, per, lanes, 11 + r, _ptr(sink), s))
+        e1.record(st)
+        torch.cuda.synchronize()
+        secs = e0.elapsed_time(e1) / 1e3
+        nl = lanes * per * reps / (8 if full else 1)
+        out[name] = {"lines_per_s": round(nl / secs, 1), "GBps_128B_lines": round(nl * 128 / secs / 1e9, 1)}
+    out["GBps_128B_lines"] = max(out["one_16B_read_per_line"]["GBps_128B_lines"],
+                                 out["whole_line_8_lanes"]["GBps_128B_lines"])
+    del buf
+    torch.cuda.empty_cache()
+    log(f"random-line roof: {out}")
+    return out
+
+
+def calib_copy() -> int:
+    """A plain copy of 1 GiB, three times: reads 1 GiB and writes 1 GiB per
+    launch, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    a = torch.ones(1 << 27, dtype=torch.int64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log("copied 1 GiB three times")
     return 0
 
 

@@ -45,7 +45,15 @@ extern "C" {
                                      default: a window with a non-ACGT char answers ~0 */
 #define SBWT_READS_INDEPENDENT 2u /* search every k-mer from the presearch table
                                      (the reference's way); default: stream
-                                     consecutive k-mers through suffix groups */
+                                     consecutive k-mers through suffix groups
+                                     (when the index has them) and skip k-mers
+                                     that contain a substring proven absent */
+#define SBWT_READS_PERSISTENT 8u  /* persistent lane-refill kernel (each lane a
+                                     state machine issuing one rank load per
+                                     iteration); default: one lane per work
+                                     item, k-mers in order */
+#define SBWT_READS_NO_CERT 16u    /* no dead-substring certificates (for
+                                     measurement) */
 #define SBWT_COUNT 4u             /* counted variant: add the 128-byte lines the
                                      rank reads touch to the index's counters
                                      (sbwt_counters); also accepted by
@@ -192,6 +200,18 @@ int sbwt_search_reads(const sbwt_index *ix, const uint64_t *d_packed,
                       const uint64_t *d_offsets, const uint64_t *d_work,
                       uint64_t rows, uint64_t work_bound, uint32_t seg,
                       uint64_t *d_out, uint32_t flags, void *stream);
+
+/* ---- measurement ------------------------------------------------------ */
+
+/* Random-access roof: `lanes` lanes each read `per_lane` (multiple of 8)
+ * uniformly random lines of line_bytes from a 128-byte-aligned buffer of
+ * `bytes`, 16 bytes per lane per line (full_line != 0: line_bytes/16 lanes
+ * read one line together). Time it with events on `stream`; lines fetched =
+ * lanes * per_lane / (full_line ? line_bytes/16 : 1). d_sink gets one u64 per
+ * lane. No reference counterpart. */
+int sbwt_gather_roof(const void *d_buf, uint64_t bytes, uint32_t line_bytes,
+                     uint32_t full_line, uint32_t per_lane, uint64_t lanes,
+                     uint64_t seed, uint64_t *d_sink, void *stream);
 
 /* ---- host-side query ingest and result text ------------------------ */
 

@@ -14,6 +14,8 @@ ERRORS = {1: "SBWT_ERR_ARG", 2: "SBWT_ERR_HIP", 3: "SBWT_ERR_IO", 4: "SBWT_ERR_F
 READS_N_AS_A = 1
 READS_INDEPENDENT = 2
 COUNT = 4
+READS_PERSISTENT = 8
+READS_NO_CERT = 16
 
 # every function the header declares, with (restype, argtypes)
 vp, u64, u32, i32 = C.c_void_p, C.c_uint64, C.c_uint32, C.c_int
@@ -55,6 +57,7 @@ SIGNATURES = {
     "sbwt_reads_work_bound": (u64, [u64, u64, u32]),
     "sbwt_reads_work": (i32, [vp, u64, u32, vp, vp]),
     "sbwt_search_reads": (i32, [vp, vp, vp, vp, vp, vp, u64, u64, u32, vp, u32, vp]),
+    "sbwt_gather_roof": (i32, [vp, u64, u32, u32, u32, u64, u64, vp, vp]),
     "sbwt_parse_lines": (i32, [cp, u64, vp, vp, u64, u64p]),
     "sbwt_format_results": (i32, [vp, vp, u64, vp, u64, u64p]),
     "sbwt_write_results": (i32, [vp, vp, u64, i32]),

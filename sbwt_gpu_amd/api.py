@@ -193,13 +193,15 @@ class Index:
         return positions
 
     def search_reads(self, reads: "ReadBatch", out: torch.Tensor | None = None, n_as_a: bool = False,
-                     independent: bool = False, count: bool = False) -> torch.Tensor:
+                     independent: bool = False, count: bool = False, persistent: bool = False,
+                     certificates: bool = True) -> torch.Tensor:
         if reads.k != self.k:
             raise ValueError(f"reads prepared for k={reads.k}, index has k={self.k}")
         if out is None:
             out = torch.empty(max(reads.total_kmers, 1), dtype=torch.int64, device=reads.chars.device)
         flags = ((_capi.READS_N_AS_A if n_as_a else 0) | (_capi.READS_INDEPENDENT if independent else 0)
-                 | (_capi.COUNT if count else 0))
+                 | (_capi.COUNT if count else 0) | (_capi.READS_PERSISTENT if persistent else 0)
+                 | (0 if certificates else _capi.READS_NO_CERT))
         check(self.lib.sbwt_search_reads(self._h, _ptr(reads.packed), _ptr(reads.nmask), _ptr(reads.row_starts),
                                          _ptr(reads.offsets), _ptr(reads.work), reads.rows, reads.work_bound,
                                          reads.seg, _ptr(out), flags, _stream()))

@@ -50,6 +50,7 @@ void destroy_index(sbwt_index* ix) {
   if (ix->d_lines) (void)hipFree(ix->d_lines);
   if (ix->d_pre) (void)hipFree(ix->d_pre);
   if (ix->d_counters) (void)hipFree(ix->d_counters);
+  if (ix->d_item_ctr) (void)hipFree(ix->d_item_ctr);
   delete ix;
 }
 
@@ -216,6 +217,8 @@ __global__ void k_clear_tail(u64* rows, u64 nwords, u64 n, u32 mark_root) {
 
 int finish_index(sbwt_index* ix, hipStream_t s) {
   SBWT_HIP(hipMalloc(&ix->d_counters, kCounters * sizeof(unsigned long long)));
+  SBWT_HIP(hipMalloc(&ix->d_item_ctr, sizeof(unsigned long long)));
+  SBWT_HIP(hipDeviceGetAttribute(&ix->num_cus, hipDeviceAttributeMultiprocessorCount, ix->device));
   SBWT_HIP(hipMemsetAsync(ix->d_counters, 0, kCounters * sizeof(unsigned long long), s));
   {
     dim3 grid(static_cast<u32>((ix->nwords + 255) / 256), kRows);

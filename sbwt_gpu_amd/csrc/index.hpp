@@ -21,6 +21,9 @@ struct sbwt_index {
   uint2* d_pre = nullptr;
   // access counters of the counted search variants (lines, steps, k-mers, full searches)
   unsigned long long* d_counters = nullptr;
+  // work-item counter of the persistent read search (zeroed by each launch)
+  unsigned long long* d_item_ctr = nullptr;
+  int num_cus = 0;
   sbwt_config cfg{};
   sbwt::u64 num_kmers = 0;  // |K| when built on the GPU, else 0
 };
@@ -46,5 +49,9 @@ void destroy_index(sbwt_index* ix);
 sbwt_index* new_index(const sbwt_config* cfg, u32 k, u64 n);
 // Validates n and k against what the device layout supports.
 int check_shape(u64 n, u32 k);
+// The persistent read search (stream.hip).
+int search_reads_v2(const sbwt_index* ix, const u64* packed, const u64* nmask, const u64* rs, const u64* off,
+                    const u64* work, u64 rows, u64 bound, u32 seg, u64* out, bool nbreak, bool stream, bool cert,
+                    bool count, hipStream_t s);
 
 }  // namespace sbwt

@@ -51,6 +51,45 @@ __device__ __forceinline__ u64 full_search(const DevIndex& ix, u64 kmer, Tally& 
   return l > r ? kNone : static_cast<u64>(l);
 }
 
+// Searches the first len (<= 32) characters of a top-aligned window. Returns
+// the interval's left end if it is non-empty after len characters, else ~0
+// with `dead` = the number of characters after which it was empty (the
+// prefix of that length is a dead string: no node label ends with it).
+template <int CB, bool COUNT>
+__device__ __forceinline__ u64 search_len(const DevIndex& ix, u64 win, u32 len, Tally& t, u32& dead) {
+  u32 l, r, i;
+  if (COUNT) t.full++;
+  if (ix.m && len >= ix.m) {
+    const uint2 e = ix.pre[win >> (64 - 2 * ix.m)];
+    l = e.x;
+    r = e.y;
+    i = ix.m;
+    if (COUNT) t.lines++;
+    if (l > r) {
+      dead = i;
+      return kNone;
+    }
+  } else {
+    l = 0;
+    r = ix.C[4] - 1u;
+    i = 0;
+  }
+  for (; i < len; i++) {
+    const u32 c = static_cast<u32>(win >> (62 - 2 * i)) & 3u;
+    if (COUNT) {
+      t.lines += line_of<CB>(l) == line_of<CB>(r + 1u) ? 1u : 2u;
+      t.steps++;
+    }
+    narrow<CB>(ix, c, l, r);
+    if (l > r) {
+      dead = i + 1;
+      return kNone;
+    }
+  }
+  dead = 0;
+  return static_cast<u64>(l);
+}
+
 // Next k-mer after a found one at node j, appending character c.
 template <int CB, bool COUNT>
 __device__ __forceinline__ u64 stream_step(const DevIndex& ix, u32 j, u32 c, Tally& t) {
@@ -130,8 +169,17 @@ __global__ __launch_bounds__(256) void k_search_positions(DevIndex ix, const u64
   pos_res[q] = full_search<CB, false>(ix, extract2(packed, pos_res[q]), t);
 }
 
-// One lane per work item: up to `seg` consecutive k-mers of one row.
-template <int CB, bool STREAM, bool NBREAK, bool COUNT>
+// One lane per work item: up to `seg` consecutive k-mers of one row, in
+// order. With STREAM, a k-mer after a found one takes one streaming step.
+// With CERT, a k-mer found absent with a dead string ending at read
+// position d (d = the new character for a failed streaming step, else where
+// the search died) triggers two short searches that can answer the next
+// absent k-mers without searching them: the k-mer starting at d (dead after
+// L characters: every k-mer containing [d, d+L-1] is absent) and the string
+// [lo-1, d] that the k-mers in between share. A dead string w proves every
+// k-mer containing w absent, because every substring of an indexed k-mer
+// ends some node label (follow the k-mer's incoming edges back to the root).
+template <int CB, bool STREAM, bool NBREAK, bool CERT, bool COUNT>
 __global__ __launch_bounds__(256) void k_search_reads(DevIndex ix, const u64* __restrict__ packed,
                                                       const u64* __restrict__ nmask,
                                                       const u64* __restrict__ row_starts,
@@ -149,28 +197,56 @@ __global__ __launch_bounds__(256) void k_search_reads(DevIndex ix, const u64* __
       if (work[mid] <= gid) lo = mid; else hi = mid;
     }
     const u64 r = lo;
-    const u64 nk = offsets[r + 1] - offsets[r];
+    const u64 rowk = offsets[r + 1] - offsets[r];
     const u64 j0 = (gid - work[r]) * seg;
-    const u64 cnt = (nk - j0) < seg ? (nk - j0) : seg;
-    const u64 c0 = row_starts[r] + j0;
-    u64* o = out + offsets[r] + j0;
+    const u64 cnt = (rowk - j0) < seg ? (rowk - j0) : seg;
+    const u64 rs = row_starts[r];
+    u64* o = out + offsets[r];
     const u32 k = ix.k;
     bool prev = false;
     u32 pnode = 0;
-    for (u64 j = 0; j < cnt; j++) {
-      const u64 pos = c0 + j;
-      const u64 kmer = extract2(packed, pos);
+    u64 clo = 1, chi = 0;  // certified-absent k-mers of the row
+    u64 kpos = ~0ull;      // a k-mer already found by a certificate search
+    u32 knode = 0;
+    for (u64 j = j0; j < j0 + cnt; j++) {
       u64 res;
-      if (NBREAK && (extract1(nmask, pos) >> (64 - k)) != 0) {
+      u64 d = 0;  // end of a dead string found at this k-mer (0: none)
+      if (CERT && j >= clo && j <= chi) {
         res = kNone;
+      } else if (NBREAK && (extract1(nmask, rs + j) >> (64 - k)) != 0) {
+        res = kNone;
+      } else if (CERT && j == kpos) {
+        res = knode;
       } else if (STREAM && prev) {
-        res = stream_step<CB, COUNT>(ix, pnode, static_cast<u32>(kmer >> (64 - 2 * k)) & 3u, t);
+        res = stream_step<CB, COUNT>(ix, pnode, static_cast<u32>(extract2(packed, rs + j) >> (64 - 2 * k)) & 3u, t);
+        if (res == kNone) d = j + k - 1;
       } else {
-        res = full_search<CB, COUNT>(ix, kmer, t);
+        u32 dead;
+        res = search_len<CB, COUNT>(ix, extract2(packed, rs + j), k, t, dead);
+        if (res == kNone) d = j + dead - 1;
       }
       prev = res != kNone;
       pnode = static_cast<u32>(res);
       o[j] = res;
+      if (CERT && d > j && d < rowk && !(clo <= chi && chi > j)) {
+        u32 lp;
+        const u64 rp = search_len<CB, COUNT>(ix, extract2(packed, rs + d), k, t, lp);
+        if (rp != kNone) {
+          kpos = d;
+          knode = static_cast<u32>(rp);
+        } else {
+          const u64 lo2 = d + lp > k ? d + lp - k : 0;
+          clo = lo2 > j + 1 ? lo2 : j + 1;
+          chi = d;
+          if (lo2 > j + 1 && d - lo2 + 2 <= k) {
+            // the k-mers j+1 .. lo2-1 all contain [lo2-1, d]
+            const u64 q = lo2 - 1;
+            u32 dl;
+            if (search_len<CB, COUNT>(ix, extract2(packed, rs + q), static_cast<u32>(d - q + 1), t, dl) == kNone)
+              clo = j + 1;
+          }
+        }
+      }
     }
     done = static_cast<u32>(cnt);
   }
@@ -205,27 +281,34 @@ struct ReadArgs {
   u64* out;
 };
 
-template <int CB, bool STREAM, bool NBREAK, bool COUNT>
-static int launch_reads4(const sbwt_index* ix, const ReadArgs& a, hipStream_t s) {
+template <int CB, bool STREAM, bool NBREAK, bool CERT, bool COUNT>
+static int launch_reads5(const sbwt_index* ix, const ReadArgs& a, hipStream_t s) {
   const u32 bt = 256;
-  k_search_reads<CB, STREAM, NBREAK, COUNT><<<(a.bound + bt - 1) / bt, bt, 0, s>>>(
+  k_search_reads<CB, STREAM, NBREAK, CERT, COUNT><<<(a.bound + bt - 1) / bt, bt, 0, s>>>(
       dev_view(ix), a.packed, a.nmask, a.rs, a.off, a.work, a.rows, a.bound, a.seg, a.out, ix->d_counters);
   SBWT_HIP(hipGetLastError());
   return SBWT_OK;
 }
 
+template <int CB, bool STREAM, bool NBREAK, bool CERT>
+static int launch_reads4(const sbwt_index* ix, const ReadArgs& a, bool count, hipStream_t s) {
+  return count ? launch_reads5<CB, STREAM, NBREAK, CERT, true>(ix, a, s)
+               : launch_reads5<CB, STREAM, NBREAK, CERT, false>(ix, a, s);
+}
+
 template <int CB, bool STREAM, bool NBREAK>
-static int launch_reads3(const sbwt_index* ix, const ReadArgs& a, bool count, hipStream_t s) {
-  return count ? launch_reads4<CB, STREAM, NBREAK, true>(ix, a, s) : launch_reads4<CB, STREAM, NBREAK, false>(ix, a, s);
+static int launch_reads3(const sbwt_index* ix, const ReadArgs& a, bool cert, bool count, hipStream_t s) {
+  return cert ? launch_reads4<CB, STREAM, NBREAK, true>(ix, a, count, s)
+              : launch_reads4<CB, STREAM, NBREAK, false>(ix, a, count, s);
 }
 
 template <int CB>
-static int launch_reads(const sbwt_index* ix, const ReadArgs& a, bool stream, bool nbreak, bool count,
+static int launch_reads(const sbwt_index* ix, const ReadArgs& a, bool stream, bool nbreak, bool cert, bool count,
                         hipStream_t s) {
-  if (stream && nbreak) return launch_reads3<CB, true, true>(ix, a, count, s);
-  if (stream) return launch_reads3<CB, true, false>(ix, a, count, s);
-  if (nbreak) return launch_reads3<CB, false, true>(ix, a, count, s);
-  return launch_reads3<CB, false, false>(ix, a, count, s);
+  if (stream && nbreak) return launch_reads3<CB, true, true>(ix, a, cert, count, s);
+  if (stream) return launch_reads3<CB, true, false>(ix, a, cert, count, s);
+  if (nbreak) return launch_reads3<CB, false, true>(ix, a, cert, count, s);
+  return launch_reads3<CB, false, false>(ix, a, cert, count, s);
 }
 
 }  // namespace sbwt
@@ -276,20 +359,23 @@ int sbwt_search_reads(const sbwt_index* ix, const uint64_t* d_packed, const uint
   if (rows == 0 || work_bound == 0) return SBWT_OK;
   SBWT_REQUIRE(d_packed && d_row_starts && d_offsets && d_work && d_out, "device buffers must not be NULL");
   const bool nbreak = !(flags & SBWT_READS_N_AS_A);
-  const bool stream_mode = !(flags & SBWT_READS_INDEPENDENT);
+  const bool independent = (flags & SBWT_READS_INDEPENDENT) != 0;
   SBWT_REQUIRE(!nbreak || d_nmask, "d_nmask is required unless SBWT_READS_N_AS_A is set");
-  if (stream_mode && !ix->has_groups)
-    return fail(SBWT_ERR_UNSUPPORTED,
-                "streaming search needs the suffix-group row; open the index with it or pass "
-                "SBWT_READS_INDEPENDENT");
   if (seg == 0) seg = 256;
   const bool cnt = (flags & SBWT_COUNT) != 0;
+  // independent: every k-mer searched from the presearch table, nothing
+  // streamed or certified (the reference's way)
+  const bool stream_mode = !independent && ix->has_groups;
+  const bool cert = !independent && !(flags & SBWT_READS_NO_CERT);
+  if (flags & SBWT_READS_PERSISTENT)
+    return search_reads_v2(ix, d_packed, d_nmask, d_row_starts, d_offsets, d_work, rows, work_bound, seg, d_out,
+                           nbreak, stream_mode, cert, cnt, as_stream(stream));
   const ReadArgs a{d_packed, d_nmask, d_row_starts, d_offsets, d_work, rows, work_bound, seg, d_out};
   hipStream_t s = as_stream(stream);
   switch (ix->layout) {
-    case 16: return launch_reads<16>(ix, a, stream_mode, nbreak, cnt, s);
-    case 32: return launch_reads<32>(ix, a, stream_mode, nbreak, cnt, s);
-    default: return launch_reads<64>(ix, a, stream_mode, nbreak, cnt, s);
+    case 16: return launch_reads<16>(ix, a, stream_mode, nbreak, cert, cnt, s);
+    case 32: return launch_reads<32>(ix, a, stream_mode, nbreak, cert, cnt, s);
+    default: return launch_reads<64>(ix, a, stream_mode, nbreak, cert, cnt, s);
   }
 }
 

@@ -176,6 +176,10 @@ def _reads(rng, g, n_reads, err=0.01, nrate=0.0, lens=(150, 150)):
     return out
 
 
+READ_VARIANTS = [{}, {"certificates": False}, {"persistent": True}, {"persistent": True, "certificates": False},
+                 {"independent": True}, {"independent": True, "persistent": True}]
+
+
 @pytest.mark.parametrize("layout", [16, 32, 64])
 @pytest.mark.parametrize("seg", [256, 7, 1])
 def test_search_reads_matches_oracle(layout, seg):
@@ -192,21 +196,76 @@ def test_search_reads_matches_oracle(layout, seg):
                          seg=seg, host_starts=rs)
     for n_as_a in (False, True):
         want = O.search_reads(oix, chars, rs, k, n_breaks=not n_as_a)
-        for independent in (False, True):
-            got = sg.to_u64(ix.search_reads(batch, n_as_a=n_as_a, independent=independent))
-            np.testing.assert_array_equal(got, want, err_msg=f"n_as_a={n_as_a} independent={independent}")
+        for kw in READ_VARIANTS:
+            got = sg.to_u64(ix.search_reads(batch, n_as_a=n_as_a, **kw))
+            np.testing.assert_array_equal(got, want, err_msg=f"n_as_a={n_as_a} {kw}")
 
 
-def test_streaming_needs_group_row():
+def test_index_without_group_row():
+    """Without suffix-group starts every k-mer is searched (certificates still
+    apply)."""
     rng = np.random.default_rng(2)
     g = genome(rng, 5000)
     s = O.SBWT([g], 15)
     ix = sg.Index.from_rows(s.rows, s.n, 15, config=sg.Config(presearch_depth=6))
-    batch = sg.ReadBatch.from_host([g[:200]], 15)
-    with pytest.raises(sg.SbwtError):
-        ix.search_reads(batch)
-    got = sg.to_u64(ix.search_reads(batch, independent=True))
-    assert (got != NONE).all()
+    reads = [g[:200], g[1000:1100] + "ACGTTT" + g[2000:2100], "".join(ACGT[rng.integers(0, 4, 300)])]
+    batch = sg.ReadBatch.from_host(reads, 15)
+    chars, rs = O.parse_lines("\n".join(reads).encode())
+    oix = oracle_index(s, 6)
+    want = O.search_reads(oix, chars, rs, 15)
+    for kw in READ_VARIANTS:
+        np.testing.assert_array_equal(sg.to_u64(ix.search_reads(batch, **kw)), want)
+
+
+@pytest.mark.parametrize("k,m", [(31, 12), (31, 0), (20, 8), (32, 10), (9, 4)])
+def test_certificates_on_negative_heavy_reads(k, m):
+    """Reads from an unrelated genome (almost every k-mer absent), reads with
+    dense errors, short rows and rows of exactly k: every variant agrees with
+    the oracle."""
+    rng = np.random.default_rng(k * 100 + m)
+    g = genome(rng, 150_000)
+    other = genome(rng, 50_000)
+    s = O.SBWT([g], k)
+    reads = _reads(rng, other, 150, err=0.0, lens=(k - 1, 300))
+    reads += _reads(rng, g, 150, err=0.05, lens=(k, 300))
+    reads += _reads(rng, g, 100, err=0.01, nrate=0.03, lens=(k, 500))
+    reads += [g[:k], g[5:5 + k + 1], other[:k]]
+    order = rng.permutation(len(reads))
+    reads = [reads[i] for i in order]
+    chars, rs = O.parse_lines("\n".join(reads).encode())
+    oix = oracle_index(s, m)
+    want = O.search_reads(oix, chars, rs, k)
+    want_a = O.search_reads(oix, chars, rs, k, n_breaks=False)
+    ix = sg.Index.from_rows(s.rows, s.n, k, group=s.group, config=sg.Config(presearch_depth=m))
+    for seg in (256, 13):
+        batch = sg.ReadBatch.from_host(reads, k, seg=seg)
+        for kw in READ_VARIANTS:
+            np.testing.assert_array_equal(sg.to_u64(ix.search_reads(batch, **kw)), want, err_msg=str(kw))
+        np.testing.assert_array_equal(sg.to_u64(ix.search_reads(batch, n_as_a=True)), want_a)
+
+
+def test_certificates_cut_work():
+    """On reads with errors the certificates answer most absent k-mers
+    without a search: fewer searches and lines than streaming alone."""
+    rng = np.random.default_rng(77)
+    g = genome(rng, 400_000)
+    k = 31
+    s = O.SBWT([g], k)
+    ix = sg.Index.from_rows(s.rows, s.n, k, group=s.group, config=sg.Config(presearch_depth=10))
+    batch = sg.ReadBatch.from_host(_reads(rng, g, 3000, err=0.02, lens=(150, 150)), k)
+    ix.counters(reset=True)
+    a = sg.to_u64(ix.search_reads(batch, count=True))
+    with_cert = ix.counters(reset=True)
+    b = sg.to_u64(ix.search_reads(batch, count=True, certificates=False))
+    without = ix.counters(reset=True)
+    np.testing.assert_array_equal(a, b)
+    assert with_cert["kmers"] == without["kmers"] == batch.total_kmers
+    assert with_cert["lines"] < 0.8 * without["lines"]
+    ix.counters(reset=True)
+    c = sg.to_u64(ix.search_reads(batch, count=True, persistent=True))
+    pers = ix.counters(reset=True)
+    np.testing.assert_array_equal(a, c)
+    assert pers["kmers"] == batch.total_kmers and pers["lines"] < 0.8 * without["lines"]
 
 
 def test_segmental_copies_groups():
@@ -233,8 +292,9 @@ def test_segmental_copies_groups():
     for layout in (16, 32, 64):
         ix = sg.Index.from_rows(s.rows, s.n, k, group=s.group, config=sg.Config(presearch_depth=8, layout=layout))
         batch = sg.ReadBatch.from_host(reads, k)
-        got = sg.to_u64(ix.search_reads(batch))
-        np.testing.assert_array_equal(got, want)
+        for kw in READ_VARIANTS:
+            got = sg.to_u64(ix.search_reads(batch, **kw))
+            np.testing.assert_array_equal(got, want, err_msg=str(kw))
 
 
 # ---------------------------------------------------------------------------
@@ -326,6 +386,8 @@ def test_scale_properties_streaming_equals_independent():
     a = sg.to_u64(ix.search_reads(batch))
     b = sg.to_u64(ix.search_reads(batch, independent=True))
     np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(sg.to_u64(ix.search_reads(batch, persistent=True)), a)
+    np.testing.assert_array_equal(sg.to_u64(ix.search_reads(batch, certificates=False)), a)
     # error-free reads are found everywhere, at distinct-or-equal nodes
     clean = sg.ReadBatch.from_host([g[p:p + 150] for p in rng.integers(0, len(g) - 150, 2000)], k)
     c = sg.to_u64(ix.search_reads(clean))
