@@ -62,10 +62,21 @@ def main() -> int:
     ap.add_argument("--no-roof", action="store_true", help="skip the random-gather roof measurement")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 3 times (HBM counter calibration) and exit")
+    ap.add_argument("--workload", choices=["reads", "large"], default="reads",
+                    help="reads: the 100 Mbp read search (default); large: a 3 Gbp uniform-random index "
+                         "queried with independent 31-mers, 50%% present, through the reference's search "
+                         "semantics (the HBM-bound case)")
+    ap.add_argument("--large-bp", type=int, default=3_000_000_000)
+    ap.add_argument("--queries", type=int, default=1_000_000_000)
     args = ap.parse_args()
 
     if args.calib_copy:
         return calib_copy()
+    if args.workload == "large":
+        if args.gpus != 1:
+            print("bench.py measures one GPU.", file=sys.stderr)
+            return 2
+        return run_large(args)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the index is replicated per GPU and queries shard by read set "
@@ -114,6 +125,7 @@ def main() -> int:
     work = torch.empty(rows + 1, dtype=torch.int64, device=dev)
     out = torch.empty(total, dtype=torch.int64, device=dev)
     bound = L.sbwt_reads_work_bound(rows, total, args.seg)
+    items = torch.empty(bound, dtype=torch.int32, device=dev)
     flags = {"default": 0, "nocert": _capi.READS_NO_CERT, "persistent": _capi.READS_PERSISTENT,
              "persistent-nocert": _capi.READS_PERSISTENT | _capi.READS_NO_CERT,
              "independent": _capi.READS_INDEPENDENT}[args.mode]
@@ -125,11 +137,11 @@ def main() -> int:
     def step(i=None, extra_flags=0):
         check(L.sbwt_pack(_ptr(rchars), n_chars, _ptr(packed), _ptr(nmask), s))
         check(L.sbwt_kmer_offsets(_ptr(rstarts), rows, k, _ptr(offsets), s))
-        check(L.sbwt_reads_work(_ptr(offsets), rows, args.seg, _ptr(work), s))
+        check(L.sbwt_reads_work(_ptr(offsets), rows, args.seg, _ptr(work), _ptr(items), s))
         if i is not None:
             ev[2 * i].record(stream)
-        check(L.sbwt_search_reads(ix._h, _ptr(packed), _ptr(nmask), _ptr(rstarts), _ptr(offsets), _ptr(work), rows,
-                                  bound, args.seg, _ptr(out), flags | extra_flags, s))
+        check(L.sbwt_search_reads(ix._h, _ptr(packed), _ptr(nmask), _ptr(rstarts), _ptr(offsets), _ptr(work),
+                                  _ptr(items), rows, bound, args.seg, _ptr(out), flags | extra_flags, s))
         if i is not None:
             ev[2 * i + 1].record(stream)
 
@@ -196,6 +208,135 @@ def main() -> int:
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def run_large(args) -> int:
+    """A 3 Gbp uniform-random genome (generated on the GPU from a fixed seed),
+    its SBWT built on the GPU (~3e9 columns, HBM-resident: far beyond the
+    256 MiB last-level cache), and `queries` 31-mers, even ones drawn from
+    uniform genome positions (present) and odd ones from a second random
+    sequence (absent w.h.p.), each searched independently with the
+    reference's semantics (presearch gather + up to k-m narrowing steps)."""
+    import torch
+
+    import sbwt_gpu_amd as sg
+    from sbwt_gpu_amd import _capi
+    from sbwt_gpu_amd.api import _ptr, _stream
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    k, G, Q = args.k, args.large_bp, args.queries
+    R = 64_000_000  # absent-query source
+    L = _capi.load()
+    s = _stream()
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(8)
+    t = time.time()
+    chars = torch.randint(0, 4, (G + R,), dtype=torch.uint8, device=dev, generator=gen)
+    # 2-bit code -> ASCII A, C, G, T
+    chars.mul_(2).add_(65)
+    chars.add_((chars >= 69).to(torch.uint8) * 2)
+    chars.add_((chars == 73).to(torch.uint8) * 11)
+    torch.cuda.synchronize()
+    log(f"{G + R} random bases generated on the GPU in {time.time() - t:.1f}s")
+    t = time.time()
+    ix = sg.Index.build(chars[:G], k, sg.Config(presearch_depth=args.presearch, layout=args.layout),
+                        starts=torch.tensor([0, G], dtype=torch.int64, device=dev))
+    info = ix.info
+    build_s = time.time() - t
+    log(f"index built on GPU in {build_s:.1f}s: n={info['n']} columns, {info['device_bytes'] / 2**30:.2f} GiB, "
+        f"layout L{info['layout']}, m={info['presearch_depth']}")
+    packed = torch.empty(L.sbwt_packed_words(G + R), dtype=torch.int64, device=dev)
+    _capi.check(L.sbwt_pack(_ptr(chars), G + R, _ptr(packed), None, s))
+    del chars
+    pos = torch.empty(Q, dtype=torch.int64, device=dev)
+    pos[0::2] = torch.randint(0, G - k + 1, ((Q + 1) // 2,), device=dev, generator=gen)
+    pos[1::2] = G + torch.randint(0, R - k + 1, (Q // 2,), device=dev, generator=gen)
+    kmers = torch.empty_like(pos)
+    _capi.check(L.sbwt_extract_kmers(_ptr(packed), _ptr(pos), Q, _ptr(kmers), s))
+    del pos, packed
+    out = torch.empty_like(kmers)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * args.steps)]
+    for _ in range(args.warmup):
+        _capi.check(L.sbwt_search_kmers(ix._h, _ptr(kmers), Q, _ptr(out), s))
+    torch.cuda.synchronize()
+    log(f"timing {args.steps} steps of {Q} k-mers")
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ev[2 * i].record(stream)
+        _capi.check(L.sbwt_search_kmers(ix._h, _ptr(kmers), Q, _ptr(out), s))
+        ev[2 * i + 1].record(stream)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    kern_avg_s = float(np.mean([ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(args.steps)])) / 1e3
+    value = Q * args.steps / elapsed
+    res = out.cpu().numpy().view(np.uint64)
+    found = int((res != np.uint64(0xFFFFFFFFFFFFFFFF)).sum())
+    ix.counters(reset=True)
+    ix.search_kmers(kmers, out=out, count=True)
+    torch.cuda.synchronize()
+    ctr = ix.counters(reset=True)
+    lines = int(ctr["lines"])
+    achieved = lines * 128 / kern_avg_s / 1e9
+    log(f"{value / 1e9:.3f} G k-mers/s, kernel {kern_avg_s * 1e3:.2f} ms, {lines / Q:.2f} lines/k-mer, "
+        f"{achieved:.0f} GB/s of 128-B lines, found {found}/{Q}")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None, "kernel": "k_search_kmers",
+                "kernel_avg_ms": round(kern_avg_s * 1e3, 3), "lines_per_launch": lines,
+                "bytes_per_kmer": round(lines * 128 / Q, 2), "index_bytes": info["device_bytes"]}
+    if not args.no_roof:
+        roof = gather_roof()
+        roofline["measured_random_roof"] = roof
+        roofline["frac_of_measured_roof"] = round(achieved / roof["GBps_128B_lines"], 4)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_kmers(ix, kmers, res, k, args)
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "k-mer queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic (seeded, generated on the GPU)",
+        "config": {"workload": f"large index: {G / 1e9:g} Gbp uniform-random genome, k={k}, presearch "
+                               f"{args.presearch}; {Q} independent 31-mers, 50% present",
+                   "index_columns": info["n"], "index_build_s": round(build_s, 1), "layout": f"L{info['layout']}",
+                   "found_fraction": round(found / Q, 4)},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def cpu_baseline_kmers(ix, kmers, res_gpu, k, args):
+    """The CPU oracle on a bounded sample of the same k-mers + GPU parity."""
+    from oracle import oracle as O
+
+    t = time.time()
+    rows = ix.rows()
+    oix = O.Index(rows[:4], ix.n, sb=1024)
+    oix.presearch(args.presearch)
+    log(f"oracle index + presearch built in {time.time() - t:.1f}s")
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    km_all = kmers[: 50_000_000].cpu().numpy().view(np.uint64)
+    probe = km_all[:200_000]
+    t = time.perf_counter()
+    oix.search(probe, k)
+    rate = probe.size / (time.perf_counter() - t)
+    n = int(min(km_all.size, max(200_000, rate * args.cpu_seconds)))
+    km = km_all[:n]
+    t = time.perf_counter()
+    want = oix.search(km, k)
+    dt = time.perf_counter() - t
+    parity = bool(np.array_equal(want, res_gpu[:n]))
+    km1 = km[: max(1, n // max(threads, 1) // 4)]
+    t = time.perf_counter()
+    oix.search(km1, k, threads=1)
+    rate1 = km1.size / (time.perf_counter() - t)
+    log(f"CPU oracle: {n / dt / 1e6:.2f} M k-mers/s on {threads} threads, {rate1 / 1e6:.2f} M/s on 1; "
+        f"GPU parity on the sample: {parity}")
+    return {"value": round(n / dt, 1), "unit": "k-mer queries/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} of the same k-mers, poppy SB=1024, presearch {args.presearch}, OpenMP over k-mers",
+            "single_thread_value": round(rate1, 1), "gpu_parity_on_sample": parity, "cpu_model": _cpu_model()}
 
 
 def gather_roof(gib: int = 4):

@@ -181,25 +181,54 @@ int sbwt_positions(const uint64_t *d_row_starts, const uint64_t *d_offsets,
                    uint64_t rows, uint32_t k, uint64_t *d_positions,
                    void *stream);
 
-/* Upper bound of the work items of sbwt_search_reads, and the work-item
- * prefix d_work[rows+1] for segments of seg k-mers (0 = the index's
- * configured segment length). */
+/* Top-aligned k-mers starting at count character positions of the packed
+ * string (the extraction of include/gpu/search_kernel.hpp:66-68). Each
+ * position p needs packed words up to 2p/64 + 1. */
+int sbwt_extract_kmers(const uint64_t *d_packed, const uint64_t *d_positions,
+                       uint64_t count, uint64_t *d_kmers, void *stream);
+
+/* Upper bound of the work items of sbwt_search_reads (segments of seg
+ * k-mers; 0 = 256), the work-item prefix d_work[rows+1], and, when d_items
+ * is not NULL, the row of every work item (work_bound u32 entries), which
+ * lets the persistent kernel fetch an item with one load. */
 uint64_t sbwt_reads_work_bound(uint64_t rows, uint64_t total_kmers, uint32_t seg);
 int sbwt_reads_work(const uint64_t *d_offsets, uint64_t rows, uint32_t seg,
-                    uint64_t *d_work, void *stream);
+                    uint64_t *d_work, uint32_t *d_items, void *stream);
 
 /* Searches every k-mer of every row; d_out[d_offsets[r] + j] is the answer
  * for the k-mer starting at character j of row r. Replaces the query half of
  * epic::KmerSearch::search (include/epicseq/kmer_search.hpp:97-153):
  * packing, positions and SearchCaller::search fused into one pass over the
- * rows. d_nmask may be NULL only with SBWT_READS_N_AS_A. work_bound is
+ * rows. d_nmask may be NULL only with SBWT_READS_N_AS_A; d_items may be
+ * NULL (work items are then located by binary search). work_bound is
  * sbwt_reads_work_bound of the same rows. No host synchronisation; can be
  * captured in a HIP graph. */
 int sbwt_search_reads(const sbwt_index *ix, const uint64_t *d_packed,
                       const uint64_t *d_nmask, const uint64_t *d_row_starts,
                       const uint64_t *d_offsets, const uint64_t *d_work,
-                      uint64_t rows, uint64_t work_bound, uint32_t seg,
-                      uint64_t *d_out, uint32_t flags, void *stream);
+                      const uint32_t *d_items, uint64_t rows,
+                      uint64_t work_bound, uint32_t seg, uint64_t *d_out,
+                      uint32_t flags, void *stream);
+
+/* ---- select and enumeration (no reference counterpart: the reference has
+ * rank only). Needed to enumerate the index's k-mers, e.g. as training
+ * positives of the prefilter. ----------------------------------------- */
+
+/* d_out[q] = column of the d_ranks[q]-th (0-based) 1-bit of row c (0..3),
+ * or ~0 when the row has fewer 1-bits. Builds a sampled select index on
+ * first use (held by the index). */
+int sbwt_select(sbwt_index *ix, uint32_t c, const uint64_t *d_ranks,
+                uint64_t count, uint64_t *d_out, void *stream);
+
+/* Label of every node j < n as a top-aligned k-mer in d_labels[j], with
+ * d_full[j] = 1, or ~0 and d_full[j] = 0 for '$'-padded nodes. */
+int sbwt_node_labels(sbwt_index *ix, uint64_t *d_labels, uint8_t *d_full,
+                     void *stream);
+
+/* Every k-mer of the index, in node (colex) order: d_kmers needs n entries;
+ * *count_out gets |K|. Synchronizes the stream. */
+int sbwt_enumerate_kmers(sbwt_index *ix, uint64_t *d_kmers, uint64_t *count_out,
+                         void *stream);
 
 /* ---- measurement ------------------------------------------------------ */
 

@@ -44,7 +44,7 @@ struct Parameters {
   uint32_t layout = SBWT_LAYOUT_L16;
   uint32_t segment_kmers = 256;
   bool store_results = true;
-  bool streaming = true;      // stream consecutive k-mers (needs the group file)
+  bool independent = false;   // search every k-mer from the presearch table
   bool n_as_a = false;        // reference packing of non-ACGT characters
   std::string filename_A, filename_C, filename_G, filename_T, filename_groups, fileQueries;
 
@@ -74,7 +74,7 @@ struct Parameters {
       else if (take(a, "--file-queries", v)) fileQueries = v;
       else if (!strcmp(a, "--without-presearch")) with_presearch = false;
       else if (!strcmp(a, "--do-not-store-results")) store_results = false;
-      else if (!strcmp(a, "--independent")) streaming = false;
+      else if (!strcmp(a, "--independent")) independent = true;
       else if (!strcmp(a, "--n-as-a")) n_as_a = true;
       else {
         fprintf(stderr, "Unknown argument %s.\n", a);
@@ -87,7 +87,6 @@ struct Parameters {
     }
     if (!with_presearch) k_presearch = 0;
     if (k_presearch > k) k_presearch = k;
-    if (filename_groups.empty()) streaming = false;
     return 0;
   }
 };
@@ -183,6 +182,7 @@ class KmerSearch {
     uint8_t* d_chars = nullptr;
     uint64_t *d_starts = nullptr, *d_packed = nullptr, *d_nmask = nullptr, *d_off = nullptr, *d_work = nullptr,
              *d_out = nullptr;
+    uint32_t* d_items = nullptr;
     int st = 1;
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
@@ -191,7 +191,7 @@ class KmerSearch {
       if (hipMalloc(&d_chars, n_chars ? n_chars : 1) || hipMalloc(&d_starts, (rows + 1) * 8) ||
           hipMalloc(&d_packed, sbwt_packed_words(n_chars) * 8) || hipMalloc(&d_nmask, sbwt_nmask_words(n_chars) * 8) ||
           hipMalloc(&d_off, (rows + 1) * 8) || hipMalloc(&d_work, (rows + 1) * 8) ||
-          hipMalloc(&d_out, number_of_positions * 8)) {
+          hipMalloc(&d_out, number_of_positions * 8) || hipMalloc(&d_items, bound * 4)) {
         fprintf(stderr, "Allocating device memory for the queries failed.\n");
         break;
       }
@@ -203,12 +203,12 @@ class KmerSearch {
       void* s = dm.stream;
       if (report(sbwt_pack(d_chars, n_chars, d_packed, d_nmask, s), "pack") ||
           report(sbwt_kmer_offsets(d_starts, rows, p_.k, d_off, s), "offsets") ||
-          report(sbwt_reads_work(d_off, rows, seg, d_work, s), "work"))
+          report(sbwt_reads_work(d_off, rows, seg, d_work, d_items, s), "work"))
         break;
-      const uint32_t flags = (p_.n_as_a ? SBWT_READS_N_AS_A : 0u) | (p_.streaming ? 0u : SBWT_READS_INDEPENDENT);
+      const uint32_t flags = (p_.n_as_a ? SBWT_READS_N_AS_A : 0u) | (p_.independent ? SBWT_READS_INDEPENDENT : 0u);
       (void)hipEventRecord(e0, dm.stream);
-      if (report(sbwt_search_reads(dm.index, d_packed, d_nmask, d_starts, d_off, d_work, rows, bound, seg, d_out,
-                                   flags, s),
+      if (report(sbwt_search_reads(dm.index, d_packed, d_nmask, d_starts, d_off, d_work, d_items, rows, bound, seg,
+                                   d_out, flags, s),
                  "SearchCaller::search"))
         break;
       (void)hipEventRecord(e1, dm.stream);
@@ -222,7 +222,7 @@ class KmerSearch {
     } while (false);
     for (void* ptr : {static_cast<void*>(d_chars), static_cast<void*>(d_starts), static_cast<void*>(d_packed),
                       static_cast<void*>(d_nmask), static_cast<void*>(d_off), static_cast<void*>(d_work),
-                      static_cast<void*>(d_out)})
+                      static_cast<void*>(d_out), static_cast<void*>(d_items)})
       if (ptr) (void)hipFree(ptr);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

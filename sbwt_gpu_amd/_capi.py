@@ -54,9 +54,13 @@ SIGNATURES = {
     "sbwt_pack": (i32, [vp, u64, vp, vp, vp]),
     "sbwt_kmer_offsets": (i32, [vp, u64, u32, vp, vp]),
     "sbwt_positions": (i32, [vp, vp, u64, u32, vp, vp]),
+    "sbwt_extract_kmers": (i32, [vp, vp, u64, vp, vp]),
     "sbwt_reads_work_bound": (u64, [u64, u64, u32]),
-    "sbwt_reads_work": (i32, [vp, u64, u32, vp, vp]),
-    "sbwt_search_reads": (i32, [vp, vp, vp, vp, vp, vp, u64, u64, u32, vp, u32, vp]),
+    "sbwt_reads_work": (i32, [vp, u64, u32, vp, vp, vp]),
+    "sbwt_search_reads": (i32, [vp, vp, vp, vp, vp, vp, vp, u64, u64, u32, vp, u32, vp]),
+    "sbwt_select": (i32, [vp, u32, vp, u64, vp, vp]),
+    "sbwt_node_labels": (i32, [vp, vp, vp, vp]),
+    "sbwt_enumerate_kmers": (i32, [vp, vp, u64p, vp]),
     "sbwt_gather_roof": (i32, [vp, u64, u32, u32, u32, u64, u64, vp, vp]),
     "sbwt_parse_lines": (i32, [cp, u64, vp, vp, u64, u64p]),
     "sbwt_format_results": (i32, [vp, vp, u64, vp, u64, u64p]),

@@ -187,6 +187,27 @@ class Index:
         check(self.lib.sbwt_counters(self._h, out, int(reset), _stream()))
         return {"lines": out[0], "steps": out[1], "kmers": out[2], "full_searches": out[3]}
 
+    def select(self, c: int, ranks: torch.Tensor) -> torch.Tensor:
+        """Column of the ranks[q]-th 1-bit of row c (~0 past the last)."""
+        out = torch.empty_like(ranks)
+        check(self.lib.sbwt_select(self._h, c, _ptr(ranks), ranks.numel(), _ptr(out), _stream()))
+        return out
+
+    def node_labels(self):
+        """(labels, full): top-aligned label of every node, full=False for padded nodes."""
+        n = self.n
+        labels = torch.empty(n, dtype=torch.int64, device="cuda")
+        full = torch.empty(n, dtype=torch.uint8, device="cuda")
+        check(self.lib.sbwt_node_labels(self._h, _ptr(labels), _ptr(full), _stream()))
+        return labels, full
+
+    def enumerate_kmers(self) -> torch.Tensor:
+        """Every k-mer of the index (top-aligned), in colex order."""
+        out = torch.empty(self.n, dtype=torch.int64, device="cuda")
+        cnt = C.c_uint64()
+        check(self.lib.sbwt_enumerate_kmers(self._h, _ptr(out), C.byref(cnt), _stream()))
+        return out[: cnt.value]
+
     def search_positions(self, packed: torch.Tensor, positions: torch.Tensor) -> torch.Tensor:
         """In place over `positions`, like the reference kernel."""
         check(self.lib.sbwt_search_positions(self._h, _ptr(packed), _ptr(positions), positions.numel(), _stream()))
@@ -203,8 +224,8 @@ class Index:
                  | (_capi.COUNT if count else 0) | (_capi.READS_PERSISTENT if persistent else 0)
                  | (0 if certificates else _capi.READS_NO_CERT))
         check(self.lib.sbwt_search_reads(self._h, _ptr(reads.packed), _ptr(reads.nmask), _ptr(reads.row_starts),
-                                         _ptr(reads.offsets), _ptr(reads.work), reads.rows, reads.work_bound,
-                                         reads.seg, _ptr(out), flags, _stream()))
+                                         _ptr(reads.offsets), _ptr(reads.work), _ptr(reads.items), reads.rows,
+                                         reads.work_bound, reads.seg, _ptr(out), flags, _stream()))
         return out[: reads.total_kmers]
 
 
@@ -245,7 +266,8 @@ class ReadBatch:
         self.total_kmers = total_kmers
         self.work_bound = L.sbwt_reads_work_bound(self.rows, total_kmers, seg)
         self.work = torch.empty(self.rows + 1, dtype=torch.int64, device=dev)
-        check(L.sbwt_reads_work(_ptr(self.offsets), self.rows, seg, _ptr(self.work), s))
+        self.items = torch.empty(max(self.work_bound, 1), dtype=torch.int32, device=dev)
+        check(L.sbwt_reads_work(_ptr(self.offsets), self.rows, seg, _ptr(self.work), _ptr(self.items), s))
 
     @classmethod
     def from_host(cls, seqs, k: int, seg: int = 256, device="cuda") -> "ReadBatch":

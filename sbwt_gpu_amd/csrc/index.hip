@@ -51,6 +51,7 @@ void destroy_index(sbwt_index* ix) {
   if (ix->d_pre) (void)hipFree(ix->d_pre);
   if (ix->d_counters) (void)hipFree(ix->d_counters);
   if (ix->d_item_ctr) (void)hipFree(ix->d_item_ctr);
+  if (ix->d_select) (void)hipFree(ix->d_select);
   delete ix;
 }
 

@@ -24,6 +24,10 @@ struct sbwt_index {
   // work-item counter of the persistent read search (zeroed by each launch)
   unsigned long long* d_item_ctr = nullptr;
   int num_cus = 0;
+  // sampled select index (chunk of every 4096th 1-bit per row), built on demand
+  sbwt::u32* d_select = nullptr;
+  sbwt::u32 select_layout = 0;
+  sbwt::u64 select_per_row = 0;
   sbwt_config cfg{};
   sbwt::u64 num_kmers = 0;  // |K| when built on the GPU, else 0
 };
@@ -51,7 +55,7 @@ sbwt_index* new_index(const sbwt_config* cfg, u32 k, u64 n);
 int check_shape(u64 n, u32 k);
 // The persistent read search (stream.hip).
 int search_reads_v2(const sbwt_index* ix, const u64* packed, const u64* nmask, const u64* rs, const u64* off,
-                    const u64* work, u64 rows, u64 bound, u32 seg, u64* out, bool nbreak, bool stream, bool cert,
-                    bool count, hipStream_t s);
+                    const u64* work, const u32* items, u64 rows, u64 bound, u32 seg, u64* out, bool nbreak,
+                    bool stream, bool cert, bool count, hipStream_t s);
 
 }  // namespace sbwt

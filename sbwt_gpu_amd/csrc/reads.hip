@@ -99,6 +99,24 @@ __global__ __launch_bounds__(64) void k_positions(const u64* __restrict__ rs, co
   }
 }
 
+// items[w] = the row of work item w, for every w in [work[r], work[r+1]).
+__global__ __launch_bounds__(256) void k_items(const u64* __restrict__ work, u64 rows, u32* __restrict__ items) {
+  const u64 r = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x;
+  if (r >= rows) return;
+  for (u64 w = work[r]; w < work[r + 1]; w++) items[w] = static_cast<u32>(r);
+}
+
+// The k-mer starting at each position of the packed string, top-aligned
+// (the extraction of include/gpu/search_kernel.hpp:66-68, kept separate so
+// a caller can hold k-mers instead of positions).
+__global__ __launch_bounds__(256) void k_extract(const u64* __restrict__ packed, const u64* __restrict__ pos,
+                                                 u64 count, u64* __restrict__ kmers) {
+  const u64 q = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x;
+  if (q >= count) return;
+  const u64 b = pos[q] << 1;
+  kmers[q] = (packed[b >> 6] << (b & 63)) | ((packed[(b >> 6) + 1] >> (63 - (b & 63))) >> 1);
+}
+
 }  // namespace sbwt
 
 using namespace sbwt;
@@ -141,16 +159,33 @@ int sbwt_positions(const uint64_t* d_row_starts, const uint64_t* d_offsets, uint
   return SBWT_OK;
 }
 
+int sbwt_extract_kmers(const uint64_t* d_packed, const uint64_t* d_positions, uint64_t count, uint64_t* d_kmers,
+                       void* stream) {
+  clear_error();
+  if (count == 0) return SBWT_OK;
+  SBWT_REQUIRE(d_packed && d_positions && d_kmers, "device buffers must not be NULL");
+  k_extract<<<(count + 255) / 256, 256, 0, as_stream(stream)>>>(d_packed, d_positions, count, d_kmers);
+  SBWT_HIP(hipGetLastError());
+  return SBWT_OK;
+}
+
 uint64_t sbwt_reads_work_bound(uint64_t rows, uint64_t total_kmers, uint32_t seg) {
   if (seg == 0) seg = 256;
   return rows + total_kmers / seg + 1;
 }
 
-int sbwt_reads_work(const uint64_t* d_offsets, uint64_t rows, uint32_t seg, uint64_t* d_work, void* stream) {
+int sbwt_reads_work(const uint64_t* d_offsets, uint64_t rows, uint32_t seg, uint64_t* d_work, uint32_t* d_items,
+                    void* stream) {
   clear_error();
   SBWT_REQUIRE(d_offsets && d_work, "device buffers must not be NULL");
+  SBWT_REQUIRE(rows < 0xFFFFFFFFull, "rows = %llu: at most 2^32 - 2 rows per call", (unsigned long long)rows);
   if (seg == 0) seg = 256;
-  return scan_rows(WorkCount{d_offsets, rows, seg}, rows, d_work, as_stream(stream));
+  SBWT_TRY(scan_rows(WorkCount{d_offsets, rows, seg}, rows, d_work, as_stream(stream)));
+  if (d_items && rows) {
+    k_items<<<(rows + 255) / 256, 256, 0, as_stream(stream)>>>(d_work, rows, d_items);
+    SBWT_HIP(hipGetLastError());
+  }
+  return SBWT_OK;
 }
 
 }  // extern "C"

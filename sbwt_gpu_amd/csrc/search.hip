@@ -169,6 +169,26 @@ __global__ __launch_bounds__(256) void k_search_positions(DevIndex ix, const u64
   pos_res[q] = full_search<CB, false>(ix, extract2(packed, pos_res[q]), t);
 }
 
+// The 64 bits of a packed string (bits_per_char = 2) or of a one-bit mask
+// (1) starting at a character that advances by one per call, holding the
+// two words it spans in registers: one load per 32 (or 64) characters.
+template <u32 BITS>
+struct Rolling {
+  const u64* w;
+  u64 a = 0, b = 0, idx = ~0ull - 1;
+  __device__ __forceinline__ u64 at(u64 pos) {
+    const u64 bit = pos * BITS;
+    const u64 word = bit >> 6;
+    if (word != idx) {
+      a = (word == idx + 1) ? b : w[word];
+      b = w[word + 1];
+      idx = word;
+    }
+    const u32 sh = static_cast<u32>(bit & 63);
+    return (a << sh) | ((b >> (63 - sh)) >> 1);
+  }
+};
+
 // One lane per work item: up to `seg` consecutive k-mers of one row, in
 // order. With STREAM, a k-mer after a found one takes one streaming step.
 // With CERT, a k-mer found absent with a dead string ending at read
@@ -208,21 +228,24 @@ __global__ __launch_bounds__(256) void k_search_reads(DevIndex ix, const u64* __
     u64 clo = 1, chi = 0;  // certified-absent k-mers of the row
     u64 kpos = ~0ull;      // a k-mer already found by a certificate search
     u32 knode = 0;
+    Rolling<2> code{packed};
+    Rolling<1> bad{nmask};
     for (u64 j = j0; j < j0 + cnt; j++) {
       u64 res;
       u64 d = 0;  // end of a dead string found at this k-mer (0: none)
+      const u64 win = code.at(rs + j);
       if (CERT && j >= clo && j <= chi) {
         res = kNone;
-      } else if (NBREAK && (extract1(nmask, rs + j) >> (64 - k)) != 0) {
+      } else if (NBREAK && (bad.at(rs + j) >> (64 - k)) != 0) {
         res = kNone;
       } else if (CERT && j == kpos) {
         res = knode;
       } else if (STREAM && prev) {
-        res = stream_step<CB, COUNT>(ix, pnode, static_cast<u32>(extract2(packed, rs + j) >> (64 - 2 * k)) & 3u, t);
+        res = stream_step<CB, COUNT>(ix, pnode, static_cast<u32>(win >> (64 - 2 * k)) & 3u, t);
         if (res == kNone) d = j + k - 1;
       } else {
         u32 dead;
-        res = search_len<CB, COUNT>(ix, extract2(packed, rs + j), k, t, dead);
+        res = search_len<CB, COUNT>(ix, win, k, t, dead);
         if (res == kNone) d = j + dead - 1;
       }
       prev = res != kNone;
@@ -352,8 +375,9 @@ int sbwt_search_positions(const sbwt_index* ix, const uint64_t* d_packed, uint64
 }
 
 int sbwt_search_reads(const sbwt_index* ix, const uint64_t* d_packed, const uint64_t* d_nmask,
-                      const uint64_t* d_row_starts, const uint64_t* d_offsets, const uint64_t* d_work, uint64_t rows,
-                      uint64_t work_bound, uint32_t seg, uint64_t* d_out, uint32_t flags, void* stream) {
+                      const uint64_t* d_row_starts, const uint64_t* d_offsets, const uint64_t* d_work,
+                      const uint32_t* d_items, uint64_t rows, uint64_t work_bound, uint32_t seg, uint64_t* d_out,
+                      uint32_t flags, void* stream) {
   clear_error();
   SBWT_REQUIRE(ix, "index is NULL");
   if (rows == 0 || work_bound == 0) return SBWT_OK;
@@ -368,8 +392,8 @@ int sbwt_search_reads(const sbwt_index* ix, const uint64_t* d_packed, const uint
   const bool stream_mode = !independent && ix->has_groups;
   const bool cert = !independent && !(flags & SBWT_READS_NO_CERT);
   if (flags & SBWT_READS_PERSISTENT)
-    return search_reads_v2(ix, d_packed, d_nmask, d_row_starts, d_offsets, d_work, rows, work_bound, seg, d_out,
-                           nbreak, stream_mode, cert, cnt, as_stream(stream));
+    return search_reads_v2(ix, d_packed, d_nmask, d_row_starts, d_offsets, d_work, d_items, rows, work_bound, seg,
+                           d_out, nbreak, stream_mode, cert, cnt, as_stream(stream));
   const ReadArgs a{d_packed, d_nmask, d_row_starts, d_offsets, d_work, rows, work_bound, seg, d_out};
   hipStream_t s = as_stream(stream);
   switch (ix->layout) {

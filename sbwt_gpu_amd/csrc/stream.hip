@@ -45,6 +45,7 @@ struct ReadsDev {
   const u64* row_starts;
   const u64* offsets;
   const u64* work;
+  const u32* items;
   u64 rows;
   u32 seg;
 };
@@ -72,7 +73,6 @@ __global__ __launch_bounds__(256) void k_reads_v2(DevIndex ix, ReadsDev rd, unsi
   // (kNoWord + 1 is no word index either, so a fresh lane always loads both)
   constexpr u64 kNoWord = ~0ull - 1;
   u64 wa = 0, wb = 0, widx = kNoWord;
-  u64 ma = 0, mb = 0, midx = kNoWord;
   // search state
   u64 win = 0;
   u32 s = 0, len = 0, i = 0, l = 0, r = 0, purpose = 0;
@@ -96,17 +96,7 @@ __global__ __launch_bounds__(256) void k_reads_v2(DevIndex ix, ReadsDev rd, unsi
   };
   auto n_window = [&](u32 jj) -> u64 {
     const u64 b = cbase + jj;
-    const u64 a = b >> 6;
-    if (a != midx) {
-      if (a == midx + 1) {
-        ma = mb;
-      } else {
-        ma = rd.nmask[a];
-      }
-      mb = rd.nmask[a + 1];
-      midx = a;
-    }
-    return (ma << (b & 63)) | ((mb >> (63 - (b & 63))) >> 1);
+    return (rd.nmask[b >> 6] << (b & 63)) | ((rd.nmask[(b >> 6) + 1] >> (63 - (b & 63))) >> 1);
   };
   auto start_search = [&](u32 pur, u32 start, u32 length, u64 w) {
     purpose = pur;
@@ -134,10 +124,15 @@ __global__ __launch_bounds__(256) void k_reads_v2(DevIndex ix, ReadsDev rd, unsi
   auto fetch = [&]() -> bool {
     const u64 it = atomicAdd(items, 1ull);
     if (it >= total_items) return false;
-    u64 lo = 0, hi = rd.rows;
-    while (hi - lo > 1) {
-      const u64 mid = (lo + hi) >> 1;
-      if (rd.work[mid] <= it) lo = mid; else hi = mid;
+    u64 lo = 0;
+    if (rd.items) {
+      lo = rd.items[it];
+    } else {
+      u64 hi = rd.rows;
+      while (hi - lo > 1) {
+        const u64 mid = (lo + hi) >> 1;
+        if (rd.work[mid] <= it) lo = mid; else hi = mid;
+      }
     }
     const u64 o0 = rd.offsets[lo];
     rowk = static_cast<u32>(rd.offsets[lo + 1] - o0);
@@ -150,7 +145,6 @@ __global__ __launch_bounds__(256) void k_reads_v2(DevIndex ix, ReadsDev rd, unsi
     chi = 0;
     kpos = ~0u;
     widx = kNoWord;
-    midx = kNoWord;
     return true;
   };
   auto answer = [&](u64 v) {
@@ -400,9 +394,9 @@ static int launch_v2(const sbwt_index* ix, const ReadsDev& rd, u64 bound, u64* o
 }
 
 int search_reads_v2(const sbwt_index* ix, const u64* packed, const u64* nmask, const u64* rs, const u64* off,
-                    const u64* work, u64 rows, u64 bound, u32 seg, u64* out, bool nbreak, bool stream, bool cert,
-                    bool count, hipStream_t s) {
-  const ReadsDev rd{packed, nmask, rs, off, work, rows, seg};
+                    const u64* work, const u32* items, u64 rows, u64 bound, u32 seg, u64* out, bool nbreak,
+                    bool stream, bool cert, bool count, hipStream_t s) {
+  const ReadsDev rd{packed, nmask, rs, off, work, items, rows, seg};
   switch (ix->layout) {
     case 16: return launch_v2<16>(ix, rd, bound, out, nbreak, stream, cert, count, s);
     case 32: return launch_v2<32>(ix, rd, bound, out, nbreak, stream, cert, count, s);

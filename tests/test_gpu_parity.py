@@ -397,3 +397,53 @@ def test_scale_properties_streaming_equals_independent():
     km, valid = O.read_kmers(chars, rs, k)
     d = sg.to_u64(ix.search_kmers(dev_u64(km)))
     np.testing.assert_array_equal(d, a[: d.size])
+
+
+# ---------------------------------------------------------------------------
+# select and enumeration
+
+
+@pytest.mark.parametrize("layout", [16, 32, 64])
+def test_select_against_numpy(layout):
+    rng = np.random.default_rng(layout)
+    g = genome(rng, 60_000)
+    k = 17
+    s = O.SBWT([g], k)
+    ix = sg.Index.from_rows(s.rows, s.n, k, group=s.group, config=sg.Config(presearch_depth=6, layout=layout))
+    for c in range(4):
+        bits = np.unpackbits(s.rows[c].view(np.uint8).reshape(-1, 8)[:, ::-1].reshape(-1))[: s.n]
+        ones = np.flatnonzero(bits)
+        ranks = np.arange(ones.size + 3, dtype=np.uint64)
+        got = sg.to_u64(ix.select(c, dev_u64(ranks)))
+        np.testing.assert_array_equal(got[: ones.size], ones.astype(np.uint64))
+        assert (got[ones.size:] == NONE).all()
+
+
+@pytest.mark.parametrize("seqs,k", [(["ACGTACGTTGCAAGGCT", "TTGACCA"], 4), (["CGTAC" * 4], 5), (["ACGT" * 10, "GGGGCCCC"], 1)])
+def test_node_labels_tiny(seqs, k):
+    s = O.SBWT(seqs, k)
+    ix = sg.Index.from_rows(s.rows, s.n, k, group=s.group, config=sg.Config(presearch_depth=min(k, 2)))
+    labels, full = ix.node_labels()
+    labels, full = sg.to_u64(labels), full.cpu().numpy().astype(bool)
+    for j in range(s.n):
+        lab = s.label(j)
+        assert full[j] == ("$" not in lab), (j, lab)
+        if full[j]:
+            assert int(labels[j]) == brute.encode_top(lab)
+        else:
+            assert labels[j] == NONE
+
+
+def test_enumerate_kmers_matches_oracle_builder():
+    rng = np.random.default_rng(12)
+    seqs = [genome(rng, 40_000), genome(rng, 3000)[:1000] + "N" + genome(rng, 2000)]
+    k = 31
+    s = O.SBWT(seqs, k)
+    ix = sg.Index.build(seqs, k, sg.Config(presearch_depth=10))
+    km = sg.to_u64(ix.enumerate_kmers())
+    assert km.size == s.num_kmers
+    full = s.lens == k
+    want = np.array([brute.encode_top(s.label(j)) for j in np.flatnonzero(full)], np.uint64)
+    np.testing.assert_array_equal(km, want)
+    # every enumerated k-mer is found, at its own node
+    np.testing.assert_array_equal(sg.to_u64(ix.search_kmers(dev_u64(km))), np.flatnonzero(full).astype(np.uint64))
