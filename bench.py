@@ -62,21 +62,36 @@ def main() -> int:
     ap.add_argument("--no-roof", action="store_true", help="skip the random-gather roof measurement")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 3 times (HBM counter calibration) and exit")
-    ap.add_argument("--workload", choices=["reads", "large"], default="reads",
+    ap.add_argument("--workload", choices=["reads", "large", "batched"], default="reads",
                     help="reads: the 100 Mbp read search (default); large: a 3 Gbp uniform-random index "
                          "queried with independent 31-mers, 50%% present, through the reference's search "
-                         "semantics (the HBM-bound case)")
+                         "semantics (the HBM-bound case); batched: 10,000 uneven read sets with 5%% N, "
+                         "autotuned, in HIP-graph-captured batches")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 --pmc child runs that measure HBM traffic per launch")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--sets", type=int, default=10_000)
+    ap.add_argument("--batch-chars", type=int, default=256 << 20)
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--large-bp", type=int, default=3_000_000_000)
     ap.add_argument("--queries", type=int, default=1_000_000_000)
     args = ap.parse_args()
 
     if args.calib_copy:
         return calib_copy()
-    if args.workload == "large":
+    traffic = None
+    if args.pmc_child:
+        calib_copy()
+        args.steps, args.warmup, args.no_cpu, args.no_roof = 1, 1, True, True
+    elif not args.no_traffic and args.workload in ("reads", "large") and args.gpus == 1:
+        # before this process touches the GPU: the counter runs are children
+        traffic = measure_traffic(args)
+    args.traffic = traffic
+    if args.workload in ("large", "batched"):
         if args.gpus != 1:
             print("bench.py measures one GPU.", file=sys.stderr)
             return 2
-        return run_large(args)
+        return run_large(args) if args.workload == "large" else run_batched(args)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the index is replicated per GPU and queries shard by read set "
@@ -176,7 +191,8 @@ def main() -> int:
         f"{ctr['full_searches']} full searches, found {found}/{total}; counted == timed: {same}")
 
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic_bytes(args.traffic),
+                "traffic_detail": args.traffic,
                 "kernel": "k_reads_v2" if args.mode.startswith("persistent") else "k_search_reads", "kernel_avg_ms": round(kern_avg_s * 1e3, 4),
                 "lines_per_launch": lines, "bytes_per_kmer": round(alg_bytes / total, 2),
                 "index_bytes": info["device_bytes"]}
@@ -282,8 +298,12 @@ def run_large(args) -> int:
     achieved = lines * 128 / kern_avg_s / 1e9
     log(f"{value / 1e9:.3f} G k-mers/s, kernel {kern_avg_s * 1e3:.2f} ms, {lines / Q:.2f} lines/k-mer, "
         f"{achieved:.0f} GB/s of 128-B lines, found {found}/{Q}")
+    traffic = traffic_bytes(args.traffic)
+    if traffic is not None:  # the counter runs search 250M k-mers per launch; scale to this launch
+        traffic = round(traffic * Q / args.traffic["queries_per_launch"])
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None, "kernel": "k_search_kmers",
+                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                "traffic_detail": args.traffic, "kernel": "k_search_kmers",
                 "kernel_avg_ms": round(kern_avg_s * 1e3, 3), "lines_per_launch": lines,
                 "bytes_per_kmer": round(lines * 128 / Q, 2), "index_bytes": info["device_bytes"]}
     if not args.no_roof:
@@ -305,6 +325,150 @@ def run_large(args) -> int:
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def run_batched(args) -> int:
+    """10,000 read sets of 1-5,000 reads (lognormal length, median 300 bp,
+    100 bp-20 kbp, 1% substitutions, 5% N) against the 100 Mbp index, split
+    into batches of consecutive read sets; every batch is one
+    sbwt_reads_plan_search call (packing, offsets, work prefix, search) on
+    preallocated scratch. The timed step replays one HIP graph that holds all
+    batches; the eager loop over the same batches is timed beside it."""
+    import torch
+
+    import sbwt_gpu_amd as sg
+    from sbwt_gpu_amd import autotune, synth
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    k = args.k
+    t = time.time()
+    g = synth.genome_codes(args.genome_bp, seed=3, gc=0.4, seg_frac=0.10, seg_len=(1_000, 50_000), seg_mut=0.01)
+    gcodes = torch.from_numpy(g).to(dev)
+    gchars = torch.from_numpy(synth.to_ascii(g)).to(dev)
+    ix = sg.Index.build(gchars, k, sg.Config(presearch_depth=args.presearch, layout=args.layout),
+                        starts=torch.tensor([0, g.size], dtype=torch.int64, device=dev))
+    del gchars
+    log(f"index built: n={ix.n} ({time.time() - t:.1f}s with the genome)")
+    t = time.time()
+    chars, starts, set_starts = synth.gpu_read_sets(gcodes, args.sets, seed=5)
+    torch.cuda.synchronize()
+    del gcodes
+    rows = starts.numel() - 1
+    starts_h = starts.cpu().numpy()
+    lens = np.diff(starts_h)
+    kcount = np.maximum(lens - k + 1, 0)
+    total = int(kcount.sum())
+    log(f"{args.sets} read sets, {rows} reads, {starts_h[-1]} bases, {total} k-mers generated in "
+        f"{time.time() - t:.1f}s")
+    # batches of whole read sets, at most batch_chars characters
+    ss = set_starts.cpu().numpy()
+    cuts = [0]
+    for si in range(1, ss.size):
+        if starts_h[ss[si]] - starts_h[cuts[-1]] > args.batch_chars and ss[si - 1] > cuts[-1]:
+            cuts.append(int(ss[si - 1]))
+    if cuts[-1] != rows:
+        cuts.append(rows)
+    kcum = np.zeros(rows + 1, dtype=np.int64)
+    kcum[1:] = np.cumsum(kcount)
+    batches = []
+    for b0, b1 in zip(cuts[:-1], cuts[1:]):
+        c0, c1 = int(starts_h[b0]), int(starts_h[b1])
+        rel = (starts[b0:b1 + 1] - c0).contiguous()
+        batches.append((chars[c0:c1], rel, int(kcum[b0]), int(kcum[b1])))
+    max_rows = max(b[1].numel() - 1 for b in batches)
+    max_chars = max(b[0].numel() for b in batches)
+    out = torch.empty(total, dtype=torch.int64, device=dev)
+    log(f"{len(batches)} batches (max {max_rows} rows, {max_chars} chars)")
+
+    tune = None
+    seg = args.seg
+    persistent = args.mode.startswith("persistent")
+    if not args.no_autotune:
+        sample_chars, sample_rel, _, _ = batches[0]
+        t = time.time()
+        tune = autotune.autotune(ix, sample_chars, sample_rel, segs=(64, 256, 1024), budget_s=90.0)
+        seg = tune["best"]["segment_kmers"]
+        persistent = tune["best"]["kernel"] == "persistent"
+        log(f"autotune ({time.time() - t:.1f}s, {len(tune['trials'])} trials): {tune['best']}")
+    plan = sg.ReadsPlan(max_rows, max_chars, seg)
+
+    def run_all(stream=None):
+        for bc, brel, o0, o1 in batches:
+            plan.search(ix, bc, brel, out[o0:o1] if o1 > o0 else out[:1], stream=stream, persistent=persistent)
+
+    run_all()
+    torch.cuda.synchronize()
+    eager_ref = out.clone()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run_all()
+    torch.cuda.synchronize()
+    eager_s = (time.perf_counter() - t0) / args.steps
+    # capture every batch in one graph
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=side):
+        run_all(stream=side)
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        out.fill_(0)
+        graph.replay()
+    torch.cuda.synchronize()
+    replay_ok = bool(torch.equal(out, eager_ref))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        graph.replay()
+    torch.cuda.synchronize()
+    graph_s = (time.perf_counter() - t0) / args.steps
+    replay_ok = replay_ok and bool(torch.equal(out, eager_ref))
+    value = total / graph_s
+    res = out.cpu().numpy().view(np.uint64)
+    found = int((res != np.uint64(0xFFFFFFFFFFFFFFFF)).sum())
+    # windows without N
+    log(f"graph: {value / 1e9:.3f} G k-mers/s ({graph_s * 1e3:.1f} ms/pass), eager {total / eager_s / 1e9:.3f} G/s; "
+        f"replays equal eager: {replay_ok}; found {found}/{total}")
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_rows(ix, chars, starts_h, res, k, args)
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "k-mer queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(graph_s * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic (seeded, generated on the GPU)",
+        "config": {"workload": f"batched uneven read sets: {args.sets} sets of 1-5000 reads, lognormal length "
+                               f"(median 300, sigma 0.8, 100-20000 bp), 1% substitutions, 5% N, against the "
+                               f"{args.genome_bp / 1e6:g} Mbp index; HIP-graph batches",
+                   "reads": rows, "bases": int(starts_h[-1]), "kmers_per_step": total, "batches": len(batches),
+                   "eager_kmers_per_s": round(total / eager_s, 1), "graph_replays_equal_eager": replay_ok,
+                   "autotune_best": tune["best"] if tune else None, "found_fraction": round(found / total, 4)},
+        "roofline": None, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def cpu_baseline_rows(ix, chars, starts_h, res_gpu, k, args):
+    """The CPU oracle's read search on the first rows of the batch + parity."""
+    from oracle import oracle as O
+
+    rows = ix.rows()
+    oix = O.Index(rows[:4], ix.n, sb=1024)
+    oix.presearch(ix.info["presearch_depth"])
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    nrows = min(len(starts_h) - 1, 200_000)
+    c_end = int(starts_h[nrows])
+    ch = chars[:c_end].cpu().numpy()
+    rs = starts_h[: nrows + 1].astype(np.uint64)
+    t = time.perf_counter()
+    want = O.search_reads(oix, ch, rs, k)
+    dt = time.perf_counter() - t
+    parity = bool(np.array_equal(want, res_gpu[: want.size]))
+    log(f"CPU oracle: {want.size / dt / 1e6:.2f} M k-mers/s on {threads} threads; GPU parity on {nrows} reads: "
+        f"{parity}")
+    return {"value": round(want.size / dt, 1), "unit": "k-mer queries/s", "cores": threads, "kind": "port",
+            "sample": f"first {nrows} reads ({want.size} k-mers), independent search with N-breaking, poppy "
+                      f"SB=1024", "gpu_parity_on_sample": parity, "cpu_model": _cpu_model()}
 
 
 def cpu_baseline_kmers(ix, kmers, res_gpu, k, args):
@@ -337,6 +501,66 @@ def cpu_baseline_kmers(ix, kmers, res_gpu, k, args):
     return {"value": round(n / dt, 1), "unit": "k-mer queries/s", "cores": threads, "kind": "port",
             "sample": f"first {n} of the same k-mers, poppy SB=1024, presearch {args.presearch}, OpenMP over k-mers",
             "single_thread_value": round(rate1, 1), "gpu_parity_on_sample": parity, "cpu_model": _cpu_model()}
+
+
+def measure_traffic(args):
+    """HBM bytes per launch of the search kernel from two rocprofv3 --pmc
+    child runs (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) of this
+    workload with --pmc-child (which also copies 1 GiB three times); each
+    counter is scaled by the factor that makes it read 1 GiB for that copy
+    (HIP's device-to-device copy kernel, __amd_rocclr_copyBuffer)."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return {"error": "rocprofv3 not found"}
+    kern = "k_search_kmers" if args.workload == "large" else (
+        "k_reads_v2" if args.mode.startswith("persistent") else "k_search_reads")
+    child = [sys.executable, "-u", os.path.abspath(__file__), "--pmc-child", "--workload", args.workload,
+             "--mode", args.mode, "--layout", str(args.layout), "--presearch", str(args.presearch),
+             "--seg", str(args.seg), "--genome-bp", str(args.genome_bp), "--reads", str(args.reads),
+             "--read-len", str(args.read_len), "--k", str(args.k), "--large-bp", str(args.large_bp),
+             "--queries", str(min(args.queries, 250_000_000))]
+    out = {"kernel": kern, "counters": {}}
+    env = dict(os.environ, TMPDIR=tempfile.gettempdir())
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix=f"sbwt_pmc_{ctr}_")
+        cmd = [prof, "--pmc", ctr, "-f", "csv", "-d", d, "-o", "run", "--", *child]
+        log(f"traffic: {' '.join(cmd[:8])} ... (child run)")
+        try:
+            r = subprocess.run(cmd, capture_output=True, text=True, timeout=300, env=env, cwd=tempfile.gettempdir())
+        except subprocess.TimeoutExpired:
+            return {"error": f"{ctr} run timed out"}
+        files = [os.path.join(dp, f) for dp, _, fs in os.walk(d) for f in fs if f.endswith("counter_collection.csv")]
+        if not files:
+            return {"error": f"{ctr} run produced no counters (rc {r.returncode}): {r.stderr[-300:]}"}
+        copy, search = [], []
+        with open(files[0]) as fh:
+            for row in csv.DictReader(fh):
+                name, v = row["Kernel_Name"], float(row["Counter_Value"])
+                if "__amd_rocclr_copyBuffer" in name:
+                    copy.append(v)
+                elif kern in name and ", true>" not in name.split("(")[0][-8:]:
+                    search.append(v)
+        shutil.rmtree(d, ignore_errors=True)
+        if not copy or not search:
+            return {"error": f"{ctr}: copy or search kernel missing from the counters"}
+        factor = (1 << 20) / (sum(copy) / len(copy))  # KiB of the 1 GiB copy
+        kib = sum(search) / len(search)
+        out["counters"][ctr] = {"search_KiB_per_dispatch": round(kib, 1), "copy_KiB": round(sum(copy) / len(copy), 1),
+                                "scale": round(factor, 4), "bytes": round(kib * 1024 * factor)}
+    out["queries_per_launch"] = min(args.queries, 250_000_000) if args.workload == "large" else None
+    return out
+
+
+def traffic_bytes(t):
+    """FETCH + WRITE bytes per launch (scaled), or None."""
+    if not t or "counters" not in t or len(t["counters"]) != 2:
+        return None
+    return t["counters"]["FETCH_SIZE"]["bytes"] + t["counters"]["WRITE_SIZE"]["bytes"]
 
 
 def gather_roof(gib: int = 4):

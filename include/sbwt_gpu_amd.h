@@ -263,6 +263,31 @@ int sbwt_format_results(const uint64_t *results, const uint64_t *offsets,
 int sbwt_write_results(const uint64_t *results, const uint64_t *offsets,
                        uint64_t rows, int fd);
 
+/* ---- batched read search with preallocated scratch ------------------ */
+
+typedef struct sbwt_reads_plan sbwt_reads_plan;
+
+/* Scratch for read searches of up to max_rows rows and max_chars characters
+ * per call (packed string, non-ACGT mask, k-mer offsets, work prefix, item
+ * map, scan temporaries) with segments of seg k-mers (0 = 256). Replaces
+ * the per-call allocations of the reference's query pipeline
+ * (KmerData::allocate_memory, include/epicseq/kmer_data.hpp:56-69). */
+int sbwt_reads_plan_create(uint64_t max_rows, uint64_t max_chars, uint32_t seg,
+                           sbwt_reads_plan **out);
+int sbwt_reads_plan_free(sbwt_reads_plan *plan);
+
+/* The whole read search of one batch: packing, k-mer offsets (into
+ * d_offsets_out when not NULL, rows + 1 entries), work prefix, item map and
+ * sbwt_search_reads with `flags`. d_out needs one entry per k-mer (at most
+ * n_chars). Allocates nothing and does not synchronize, so a sequence of
+ * batches can be captured in one HIP graph and replayed; calls sharing a
+ * plan must be ordered on one stream. */
+int sbwt_reads_plan_search(sbwt_reads_plan *plan, const sbwt_index *ix,
+                           const uint8_t *d_chars, uint64_t n_chars,
+                           const uint64_t *d_row_starts, uint64_t rows,
+                           uint64_t *d_out, uint64_t *d_offsets_out,
+                           uint32_t flags, void *stream);
+
 /* Message of the calling thread's last failure ("" if none). */
 const char *sbwt_last_error(void);
 

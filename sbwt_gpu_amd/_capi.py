@@ -65,6 +65,9 @@ SIGNATURES = {
     "sbwt_parse_lines": (i32, [cp, u64, vp, vp, u64, u64p]),
     "sbwt_format_results": (i32, [vp, vp, u64, vp, u64, u64p]),
     "sbwt_write_results": (i32, [vp, vp, u64, i32]),
+    "sbwt_reads_plan_create": (i32, [u64, u64, u32, C.POINTER(vp)]),
+    "sbwt_reads_plan_free": (i32, [vp]),
+    "sbwt_reads_plan_search": (i32, [vp, vp, vp, u64, vp, u64, vp, vp, u32, vp]),
     "sbwt_last_error": (cp, []),
 }
 

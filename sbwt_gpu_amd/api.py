@@ -326,3 +326,41 @@ def format_results(results: np.ndarray, offsets: np.ndarray) -> bytes:
     buf = C.create_string_buffer(size.value + 1)
     check(L.sbwt_format_results(rp, C.c_void_p(off.ctypes.data), rows, buf, size.value, C.byref(size)))
     return buf.raw[: size.value]
+
+
+def _flags(n_as_a=False, independent=False, count=False, persistent=False, certificates=True) -> int:
+    return ((_capi.READS_N_AS_A if n_as_a else 0) | (_capi.READS_INDEPENDENT if independent else 0)
+            | (_capi.COUNT if count else 0) | (_capi.READS_PERSISTENT if persistent else 0)
+            | (0 if certificates else _capi.READS_NO_CERT))
+
+
+class ReadsPlan:
+    """Preallocated scratch for batches of up to max_rows rows and max_chars
+    characters: a batch's whole query path runs without allocating or
+    synchronizing, so sequences of batches can be captured in a HIP graph."""
+
+    def __init__(self, max_rows: int, max_chars: int, seg: int = 256):
+        self.lib = _capi.load()
+        self._h = C.c_void_p()
+        check(self.lib.sbwt_reads_plan_create(max_rows, max_chars, seg, C.byref(self._h)))
+        self.max_rows, self.max_chars, self.seg = max_rows, max_chars, seg
+
+    def search(self, index: Index, chars: torch.Tensor, row_starts: torch.Tensor, out: torch.Tensor,
+               offsets: torch.Tensor | None = None, stream=None, **flags) -> torch.Tensor:
+        """chars / row_starts may be slices of larger device buffers; row_starts
+        holds offsets into `chars` (rows + 1 entries)."""
+        rows = row_starts.numel() - 1
+        check(self.lib.sbwt_reads_plan_search(self._h, index._h, _ptr(chars), chars.numel(), _ptr(row_starts), rows,
+                                              _ptr(out), _ptr(offsets), _flags(**flags), _stream(stream)))
+        return out
+
+    def close(self) -> None:
+        if self._h:
+            self.lib.sbwt_reads_plan_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

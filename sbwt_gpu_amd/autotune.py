@@ -1,0 +1,70 @@
+"""Autotuner for the read search.
+
+Times the read search of a sample batch under every combination of rank
+layout (16/32/64-byte chunks), presearch depth, segment length (k-mers per
+work item) and kernel (one lane per segment, or the persistent lane-refill
+kernel), checks that every combination gives the same answers, and leaves
+the index in the fastest layout and depth. The reference fixes these by
+flags (superblock size, presearch depth and threads per block,
+include/parameters.hpp:39-58); here they are measured.
+"""
+from __future__ import annotations
+
+import itertools
+import time
+
+import torch
+
+from .api import Index, ReadsPlan
+
+
+def autotune(index: Index, chars: torch.Tensor, row_starts: torch.Tensor, *, layouts=(16, 32, 64),
+             depths=(10, 11, 12, 13), segs=(64, 256, 1024), kernels=("lane", "persistent"), reps: int = 3,
+             max_kmers: int | None = None, budget_s: float = 60.0, n_as_a: bool = False, log=None) -> dict:
+    """chars / row_starts: a sample batch on the device (row_starts int64,
+    offsets into chars). Returns {"best": {...}, "trials": [...]} and sets
+    the index to the best layout and presearch depth."""
+    rows = row_starts.numel() - 1
+    n_chars = chars.numel()
+    cap = max_kmers if max_kmers is not None else n_chars
+    out = torch.empty(max(cap, 1), dtype=torch.int64, device=chars.device)
+    ref = None
+    trials = []
+    t_start = time.time()
+    stream = torch.cuda.current_stream()
+    k = index.k
+    depths = [m for m in depths if m <= min(k, 14)]
+    for layout, m in itertools.product(layouts, depths):
+        if time.time() - t_start > budget_s:
+            break
+        if index.info["layout"] != layout:
+            index.set_layout(layout)
+        if index.info["presearch_depth"] != m:
+            index.presearch(m)
+        for seg, kern in itertools.product(segs, kernels):
+            plan = ReadsPlan(rows, n_chars, seg)
+            flags = {"persistent": kern == "persistent", "n_as_a": n_as_a}
+            plan.search(index, chars, row_starts, out, **flags)  # warm
+            times = []
+            for _ in range(reps):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                plan.search(index, chars, row_starts, out, **flags)
+                e1.record(stream)
+                e1.synchronize()
+                times.append(e0.elapsed_time(e1))
+            if ref is None:
+                ref = out.clone()
+            elif not torch.equal(out, ref):
+                raise RuntimeError(f"autotune: answers differ for layout={layout} m={m} seg={seg} kernel={kern}")
+            ms = sorted(times)[len(times) // 2]
+            trials.append({"layout": layout, "presearch_depth": m, "segment_kmers": seg, "kernel": kern, "ms": ms})
+            if log:
+                log(f"autotune layout=L{layout} m={m} seg={seg} {kern}: {ms:.3f} ms")
+            plan.close()
+    best = min(trials, key=lambda t: t["ms"])
+    if index.info["layout"] != best["layout"]:
+        index.set_layout(best["layout"])
+    if index.info["presearch_depth"] != best["presearch_depth"]:
+        index.presearch(best["presearch_depth"])
+    return {"best": best, "trials": trials}

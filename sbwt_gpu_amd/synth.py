@@ -101,3 +101,53 @@ def lognormal_read_sets(genome_codes_: np.ndarray, sets: int, seed: int, max_rea
     set_starts = np.zeros(per_set.size + 1, dtype=np.int64)
     set_starts[1:] = np.cumsum(per_set)
     return chars, starts, set_starts
+
+
+def gpu_read_sets(genome_codes_dev, sets: int, seed: int, max_reads: int = 5000, median: float = 300.0,
+                  sigma: float = 0.8, lo: int = 100, hi: int = 20_000, sub: float = 0.01, n_rate: float = 0.05,
+                  chunk_chars: int = 1 << 25):
+    """Uneven read sets generated on the GPU (torch, fixed seed): reads per
+    set ~ U{1..max_reads}, read length ~ lognormal(ln median, sigma) clipped
+    to [lo, hi], each read from a uniform position of the genome (2-bit codes
+    on the device), substitutions at `sub`, N at `n_rate`. Returns (ASCII
+    chars uint8, row starts int64[rows+1], set starts int64[sets+1]) on the
+    device."""
+    import torch
+
+    dev = genome_codes_dev.device
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    G = genome_codes_dev.numel()
+    per_set = torch.randint(1, max_reads + 1, (sets,), generator=gen, device=dev)
+    set_starts = torch.zeros(sets + 1, dtype=torch.int64, device=dev)
+    set_starts[1:] = torch.cumsum(per_set, 0)
+    nrows = int(set_starts[-1].item())
+    lens = torch.exp(torch.randn(nrows, generator=gen, device=dev) * sigma + float(np.log(median)))
+    lens = lens.clamp(lo, hi).to(torch.int64)
+    starts = torch.zeros(nrows + 1, dtype=torch.int64, device=dev)
+    starts[1:] = torch.cumsum(lens, 0)
+    total = int(starts[-1].item())
+    pos = (torch.rand(nrows, generator=gen, device=dev, dtype=torch.float64) * (G - lens + 1)).to(torch.int64)
+    chars = torch.empty(total, dtype=torch.uint8, device=dev)
+    lut = torch.tensor(list(b"ACGT"), dtype=torch.uint8, device=dev)
+    starts_h = starts.cpu().numpy()
+    r0 = 0
+    while r0 < nrows:
+        r1 = int(np.searchsorted(starts_h, starts_h[r0] + chunk_chars, side="right")) - 1
+        r1 = max(r1, r0 + 1)
+        r1 = min(r1, nrows)
+        c0, c1 = int(starts_h[r0]), int(starts_h[r1])
+        L = lens[r0:r1]
+        base = torch.repeat_interleave(pos[r0:r1] - (starts[r0:r1] - c0), L)
+        src = base + torch.arange(c1 - c0, device=dev)
+        codes = genome_codes_dev[src]
+        flip = torch.rand(c1 - c0, generator=gen, device=dev) < sub
+        codes = torch.where(flip, (codes + torch.randint(1, 4, (c1 - c0,), generator=gen, device=dev,
+                                                           dtype=torch.uint8)) % 4, codes)
+        a = lut[codes.to(torch.int64)]
+        if n_rate > 0:
+            a = torch.where(torch.rand(c1 - c0, generator=gen, device=dev) < n_rate,
+                            torch.tensor(ord("N"), dtype=torch.uint8, device=dev), a)
+        chars[c0:c1] = a
+        r0 = r1
+    return chars, starts, set_starts
