@@ -62,17 +62,25 @@ def main() -> int:
     ap.add_argument("--no-roof", action="store_true", help="skip the random-gather roof measurement")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 3 times (HBM counter calibration) and exit")
-    ap.add_argument("--workload", choices=["reads", "large", "batched"], default="reads",
+    ap.add_argument("--workload", choices=["reads", "large", "batched", "prefilter"], default="reads",
                     help="reads: the 100 Mbp read search (default); large: a 3 Gbp uniform-random index "
                          "queried with independent 31-mers, 50%% present, through the reference's search "
                          "semantics (the HBM-bound case); batched: 10,000 uneven read sets with 5%% N, "
-                         "autotuned, in HIP-graph-captured batches")
+                         "autotuned, in HIP-graph-captured batches; prefilter: the learned negative "
+                         "prefilter (trained on the GPU) on a 90%%-negative read mix, filtered vs plain")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc child runs that measure HBM traffic per launch")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--sets", type=int, default=10_000)
     ap.add_argument("--batch-chars", type=int, default=256 << 20)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--train-positives", type=int, default=50_000_000)
+    ap.add_argument("--train-negatives", type=int, default=50_000_000)
+    ap.add_argument("--train-batch", type=int, default=65536)
+    ap.add_argument("--train-epochs", type=int, default=1)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--threshold", type=float, default=0.0, help="prefilter logit threshold")
+    ap.add_argument("--negative-fraction", type=float, default=0.9)
     ap.add_argument("--large-bp", type=int, default=3_000_000_000)
     ap.add_argument("--queries", type=int, default=1_000_000_000)
     args = ap.parse_args()
@@ -87,11 +95,11 @@ def main() -> int:
         # before this process touches the GPU: the counter runs are children
         traffic = measure_traffic(args)
     args.traffic = traffic
-    if args.workload in ("large", "batched"):
+    if args.workload in ("large", "batched", "prefilter"):
         if args.gpus != 1:
             print("bench.py measures one GPU.", file=sys.stderr)
             return 2
-        return run_large(args) if args.workload == "large" else run_batched(args)
+        return {"large": run_large, "batched": run_batched, "prefilter": run_prefilter}[args.workload](args)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the index is replicated per GPU and queries shard by read set "
@@ -448,6 +456,231 @@ def run_batched(args) -> int:
     return 0
 
 
+MLP_FLOPS_PER_KMER = 2 * (124 * 128 + 128 * 64 + 64)  # one-hot 31x4 -> 128 -> 64 -> 1
+
+
+def run_prefilter(args) -> int:
+    """The learned negative prefilter on a 90%-negative read mix. Index: the
+    100 Mbp genome of the read-search workload. Queries: 1,000,000 reads of
+    150 bp at 1% substitutions, 90% drawn from a second independent seeded
+    100 Mbp genome (same generator, other seed) and 10% from the indexed one,
+    shuffled. The MLP is trained on the GPU on SBWT-enumerated positives and
+    uniform random k-mers the plain search confirms absent; a 1%-FPR Bloom
+    filter holds the model's false negatives over every indexed k-mer.
+
+    Timed, each over the same batch resident in HBM (one step = the whole
+    query path): plain streaming search (certificates), plain independent
+    search (the reference's per-k-mer semantics), filtered (k-mer
+    extraction + fused MLP/Bloom prefilter + masked search, in streaming and
+    independent form), and the filter without its learned part (a Bloom
+    filter over all indexed k-mers). Every filtered answer vector must equal
+    the plain one bit for bit."""
+    import torch
+
+    import sbwt_gpu_amd as sg
+    from sbwt_gpu_amd import _capi, synth
+    from sbwt_gpu_amd.api import _ptr, _stream
+    from sbwt_gpu_amd.learned_filter import train_prefilter
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    k = args.k
+    L = _capi.load()
+    check = _capi.check
+    t = time.time()
+    g1 = synth.genome_codes(args.genome_bp, seed=3, gc=0.4, seg_frac=0.10, seg_len=(1_000, 50_000), seg_mut=0.01)
+    g2 = synth.genome_codes(args.genome_bp, seed=13, gc=0.4, seg_frac=0.10, seg_len=(1_000, 50_000), seg_mut=0.01)
+    log(f"two genomes of {g1.size} bp generated in {time.time() - t:.1f}s")
+    gchars = torch.from_numpy(synth.to_ascii(g1)).to(dev)
+    t = time.time()
+    ix = sg.Index.build(gchars, k, sg.Config(presearch_depth=args.presearch, layout=args.layout),
+                        starts=torch.tensor([0, g1.size], dtype=torch.int64, device=dev))
+    del gchars
+    info = ix.info
+    log(f"index built on GPU in {time.time() - t:.1f}s: n={info['n']}")
+    n_neg = int(round(args.reads * args.negative_fraction))
+    c_neg, _ = synth.reads(g2, n_neg, args.read_len, seed=14, sub=0.01)
+    c_pos, _ = synth.reads(g1, args.reads - n_neg, args.read_len, seed=4, sub=0.01)
+    perm = np.random.default_rng(15).permutation(args.reads)
+    rchars_h = np.concatenate([c_neg, c_pos]).reshape(args.reads, args.read_len)[perm].reshape(-1)
+    is_pos_read = (perm >= n_neg)
+    rstarts_h = np.arange(args.reads + 1, dtype=np.int64) * args.read_len
+    del c_neg, c_pos
+    rchars = torch.from_numpy(rchars_h).to(dev)
+    rstarts = torch.from_numpy(rstarts_h).to(dev)
+    rows = args.reads
+    total = int(np.maximum(np.diff(rstarts_h) - k + 1, 0).sum())
+
+    # ---- train the prefilter (untimed) --------------------------------------
+    t = time.time()
+    pf = train_prefilter(ix, positives=args.train_positives, negatives=args.train_negatives,
+                         epochs=args.train_epochs, batch=args.train_batch, lr=args.lr, seed=7,
+                         threshold=args.threshold, fpr=0.01, log=log)
+    train_total_s = time.time() - t
+    log(f"prefilter ready in {train_total_s:.1f}s: {pf.stats}")
+    # the method without its learned part: a Bloom filter over every indexed k-mer
+    allpos = ix.enumerate_kmers()
+    bloom_all = sg.Bloom(k, allpos.numel(), 0.01)
+    bloom_all.insert(allpos)
+    bloom_all_zero_fn = bool((bloom_all.query(allpos) == 1).all().item())
+    # logit distribution on indexed k-mers and on fresh verified negatives (threshold table)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(99)
+    from sbwt_gpu_amd.learned_filter import random_kmers
+    vneg = random_kmers(4_000_000, k, gen)
+    vneg = vneg[ix.search_kmers(vneg) == -1]
+    zpos = pf.mlp.logits(allpos)
+    zneg = pf.mlp.logits(vneg)
+    thr_table = []
+    for thr in (-4.0, -2.0, -1.0, 0.0, 1.0, 2.0, 4.0):
+        thr_table.append({"threshold": thr, "model_fn_fraction": round(float((zpos < thr).float().mean()), 4),
+                          "random_negative_pass": round(float((zneg >= thr).float().mean()), 4)})
+    del allpos, zpos
+
+    # ---- query-path buffers ------------------------------------------------
+    n_chars = rchars.numel()
+    packed = torch.empty(L.sbwt_packed_words(n_chars), dtype=torch.int64, device=dev)
+    nmask = torch.empty(L.sbwt_nmask_words(n_chars), dtype=torch.int64, device=dev)
+    offsets = torch.empty(rows + 1, dtype=torch.int64, device=dev)
+    work = torch.empty(rows + 1, dtype=torch.int64, device=dev)
+    bound = L.sbwt_reads_work_bound(rows, total, args.seg)
+    items = torch.empty(bound, dtype=torch.int32, device=dev)
+    pos = torch.empty(total, dtype=torch.int64, device=dev)
+    kmers = torch.empty(total, dtype=torch.int64, device=dev)
+    maybe = torch.empty(total, dtype=torch.uint8, device=dev)
+    out = torch.empty(total, dtype=torch.int64, device=dev)
+    s = _stream()
+    stream = torch.cuda.current_stream()
+    nsteps = max(args.steps, 1)
+
+    def prep():
+        check(L.sbwt_pack(_ptr(rchars), n_chars, _ptr(packed), _ptr(nmask), s))
+        check(L.sbwt_kmer_offsets(_ptr(rstarts), rows, k, _ptr(offsets), s))
+        check(L.sbwt_reads_work(_ptr(offsets), rows, args.seg, _ptr(work), _ptr(items), s))
+
+    def search(flags, masked):
+        a = (ix._h, _ptr(packed), _ptr(nmask), _ptr(rstarts), _ptr(offsets), _ptr(work), _ptr(items), rows, bound,
+             args.seg)
+        if masked:
+            check(L.sbwt_search_reads_masked(*a, _ptr(maybe), _ptr(out), flags, s))
+        else:
+            check(L.sbwt_search_reads(*a, _ptr(out), flags, s))
+
+    def make_path(flags, filt):
+        """filt: None (plain), "learned" (MLP + Bloom over its false
+        negatives) or "bloom" (Bloom over every indexed k-mer)."""
+        def run(ev=None):
+            prep()
+            if filt is not None:
+                check(L.sbwt_positions(_ptr(rstarts), _ptr(offsets), rows, k, _ptr(pos), s))
+                check(L.sbwt_extract_kmers(_ptr(packed), _ptr(pos), total, _ptr(kmers), s))
+                if ev:
+                    ev[0].record(stream)
+                if filt == "learned":
+                    pf.maybe(kmers, out=maybe)
+                else:
+                    check(L.sbwt_bloom_query(bloom_all._h, _ptr(kmers), total, _ptr(maybe), s))
+            if ev:
+                ev[1].record(stream)
+            search(flags, filt is not None)
+            if ev:
+                ev[2].record(stream)
+        return run
+
+    paths = {
+        "plain": make_path(0, None),
+        "plain_independent": make_path(_capi.READS_INDEPENDENT, None),
+        "filtered": make_path(0, "learned"),
+        "filtered_independent": make_path(_capi.READS_INDEPENDENT, "learned"),
+        "bloom_only": make_path(0, "bloom"),
+        "bloom_only_independent": make_path(_capi.READS_INDEPENDENT, "bloom"),
+    }
+    results, answers = {}, {}
+    for name, run in paths.items():
+        for _ in range(args.warmup):
+            run()
+        torch.cuda.synchronize()
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(nsteps)]
+        t0 = time.perf_counter()
+        for i in range(nsteps):
+            run(evs[i])
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / nsteps
+        filt_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs])) if name not in (
+            "plain", "plain_independent") else 0.0
+        search_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+        answers[name] = out.cpu().numpy().view(np.uint64).copy()
+        mrate = float(maybe.float().mean().item()) if "plain" not in name else None
+        results[name] = {"kmers_per_s": round(total / el, 1), "ms_per_step": round(el * 1e3, 3),
+                         "filter_kernel_ms": round(filt_ms, 3), "search_kernel_ms": round(search_ms, 3),
+                         "maybe_fraction": round(mrate, 4) if mrate is not None else None}
+        log(f"{name}: {total / el / 1e9:.3f} G k-mers/s ({el * 1e3:.2f} ms/step; filter {filt_ms:.2f} ms, "
+            f"search {search_ms:.2f} ms, maybe {mrate})")
+    plain = answers["plain"]
+    exact = {n: bool(np.array_equal(a, plain)) for n, a in answers.items()}
+    log(f"answers equal the plain search: {exact}")
+    none = np.uint64(0xFFFFFFFFFFFFFFFF)
+    found = int((plain != none).sum())
+    per_read = args.read_len - k + 1
+    pos_kmer = np.repeat(is_pos_read, per_read)
+    neg_frac_kmers = float(1.0 - (plain != none).mean())
+    maybe_l = pf.maybe(kmers).cpu().numpy().astype(bool)
+    m_on_absent = float(maybe_l[plain == none].mean())
+    m_on_negreads = float(maybe_l[~pos_kmer].mean())
+
+    # roofline of the filtered path's dominant kernel
+    fr = results["filtered"]
+    pf_s = fr["filter_kernel_ms"] / 1e3
+    ix.counters(reset=True)
+    pf.maybe(kmers, out=maybe)
+    search(_capi.COUNT, True)
+    torch.cuda.synchronize()
+    ctr = ix.counters(reset=True)
+    search_gbps = int(ctr["lines"]) * 128 / (fr["search_kernel_ms"] / 1e3) / 1e9
+    mlp_tflops = total * MLP_FLOPS_PER_KMER / pf_s / 1e12
+    mlp_roof = {"bound": "mfma", "achieved": round(mlp_tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
+                "frac": round(mlp_tflops / 2500.0, 4), "traffic": None, "kernel": "k_prefilter",
+                "kernel_avg_ms": fr["filter_kernel_ms"], "flops_per_kmer": MLP_FLOPS_PER_KMER}
+    search_roof = {"bound": "hbm", "achieved": round(search_gbps, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                   "frac": round(search_gbps / HBM_PEAK_GBPS, 4), "traffic": None,
+                   "kernel": "k_search_reads (masked)", "kernel_avg_ms": fr["search_kernel_ms"],
+                   "lines_per_launch": int(ctr["lines"])}
+    roofline = dict(mlp_roof if fr["filter_kernel_ms"] >= fr["search_kernel_ms"] else search_roof)
+    roofline["other_kernel"] = search_roof if roofline["kernel"] == "k_prefilter" else mlp_roof
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(ix, rchars_h, rstarts_h, plain, k, args)
+    value = results["filtered"]["kmers_per_s"]
+    line = {
+        "metric": METRIC, "value": value, "unit": "k-mer queries/s", "n_gpus": 1, "steps": nsteps,
+        "warmup": args.warmup, "ms_per_step": results["filtered"]["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32 search, bf16/fp32 MLP",
+        "data": "synthetic (seeded genomes + reads generated in-run)",
+        "config": {"workload": f"learned prefilter: {args.genome_bp / 1e6:g} Mbp index (GC 40%, 10% segmental "
+                               f"copies), k={k}, presearch {args.presearch}; {rows} reads x {args.read_len} bp at "
+                               f"1% substitutions, {args.negative_fraction:.0%} from a second independent "
+                               f"{args.genome_bp / 1e6:g} Mbp genome; MLP 124-128-64-1 bf16/fp32 trained on "
+                               f"{pf.stats['train_positives']} enumerated positives + "
+                               f"{pf.stats['train_negatives']} verified negatives; Bloom 1% FPR over its false "
+                               f"negatives",
+                   "kmers_per_step": total, "absent_fraction": round(neg_frac_kmers, 4), "found": found,
+                   "paths": results, "answers_equal_plain": exact,
+                   "filtered_vs_plain": round(results["filtered"]["kmers_per_s"] / results["plain"]["kmers_per_s"], 3),
+                   "filtered_independent_vs_plain_independent": round(
+                       results["filtered_independent"]["kmers_per_s"] / results["plain_independent"]["kmers_per_s"], 3),
+                   "learned_vs_bloom_only": round(results["filtered"]["kmers_per_s"] /
+                                                  results["bloom_only"]["kmers_per_s"], 3),
+                   "maybe_on_absent_kmers": round(m_on_absent, 4),
+                   "maybe_on_negative_read_kmers": round(m_on_negreads, 4),
+                   "threshold": args.threshold, "threshold_table": thr_table,
+                   "bloom_only_bytes": bloom_all.bytes, "bloom_only_zero_fn": bloom_all_zero_fn,
+                   "train": pf.stats, "train_total_s": round(train_total_s, 1)},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def cpu_baseline_rows(ix, chars, starts_h, res_gpu, k, args):
     """The CPU oracle's read search on the first rows of the batch + parity."""
     from oracle import oracle as O
@@ -548,9 +781,14 @@ def measure_traffic(args):
         shutil.rmtree(d, ignore_errors=True)
         if not copy or not search:
             return {"error": f"{ctr}: copy or search kernel missing from the counters"}
-        factor = (1 << 20) / (sum(copy) / len(copy))  # KiB of the 1 GiB copy
+        # the three 1 GiB copies are the largest copy dispatches of the child
+        big = sorted(copy)[-3:]
+        copy_kib = sum(big) / len(big)
+        if copy_kib < 100_000:
+            return {"error": f"{ctr}: the 1 GiB calibration copies were not found"}
+        factor = (1 << 20) / copy_kib
         kib = sum(search) / len(search)
-        out["counters"][ctr] = {"search_KiB_per_dispatch": round(kib, 1), "copy_KiB": round(sum(copy) / len(copy), 1),
+        out["counters"][ctr] = {"search_KiB_per_dispatch": round(kib, 1), "copy_KiB": round(copy_kib, 1),
                                 "scale": round(factor, 4), "bytes": round(kib * 1024 * factor)}
     out["queries_per_launch"] = min(args.queries, 250_000_000) if args.workload == "large" else None
     return out

@@ -181,6 +181,19 @@ int sbwt_positions(const uint64_t *d_row_starts, const uint64_t *d_offsets,
                    uint64_t rows, uint32_t k, uint64_t *d_positions,
                    void *stream);
 
+/* sbwt_search_reads where k-mers with d_maybe[offset] == 0 (one byte per
+ * k-mer in output order, e.g. from sbwt_prefilter) answer ~0 without a
+ * search. Uses the one-lane-per-segment kernel (SBWT_READS_PERSISTENT is
+ * ignored). The answers equal the unmasked search whenever no indexed
+ * k-mer is marked 0. */
+int sbwt_search_reads_masked(const sbwt_index *ix, const uint64_t *d_packed,
+                             const uint64_t *d_nmask, const uint64_t *d_row_starts,
+                             const uint64_t *d_offsets, const uint64_t *d_work,
+                             const uint32_t *d_items, uint64_t rows,
+                             uint64_t work_bound, uint32_t seg,
+                             const uint8_t *d_maybe, uint64_t *d_out,
+                             uint32_t flags, void *stream);
+
 /* Top-aligned k-mers starting at count character positions of the packed
  * string (the extraction of include/gpu/search_kernel.hpp:66-68). Each
  * position p needs packed words up to 2p/64 + 1. */
@@ -229,6 +242,56 @@ int sbwt_node_labels(sbwt_index *ix, uint64_t *d_labels, uint8_t *d_full,
  * *count_out gets |K|. Synchronizes the stream. */
 int sbwt_enumerate_kmers(sbwt_index *ix, uint64_t *d_kmers, uint64_t *count_out,
                          void *stream);
+
+/* ---- learned negative prefilter (no reference counterpart) ------------
+ * An MLP over one-hot 2-bit k-mers, 4k inputs -> 128 -> 64 -> 1 (ReLU,
+ * logistic output), bf16 weights with FP32 accumulation, trained on the GPU;
+ * and a blocked Bloom filter over the k-mers of the index the model rejects,
+ * so that "maybe" = logit >= threshold or Bloom hit never misses an indexed
+ * k-mer. Parameters are one flat FP32 vector: W1[4k][128], b1[128],
+ * W2[128][64], b2[64], w3[64], b3 (row = input index 4t + c). */
+typedef struct sbwt_mlp sbwt_mlp;
+typedef struct sbwt_bloom sbwt_bloom;
+
+/* A model for k-mers of length k with He-initialised weights from `seed`,
+ * and scratch for training batches of up to max_batch k-mers. */
+int sbwt_mlp_create(uint32_t k, uint64_t seed, uint64_t max_batch, sbwt_mlp **out);
+int sbwt_mlp_free(sbwt_mlp *nn);
+uint64_t sbwt_mlp_param_count(const sbwt_mlp *nn);
+/* Host copies of the FP32 master parameters (set also resets Adam). */
+int sbwt_mlp_set_params(sbwt_mlp *nn, const float *params, uint64_t count);
+int sbwt_mlp_get_params(const sbwt_mlp *nn, float *params, uint64_t count);
+/* Gradient (device, param_count floats) and mean binary cross-entropy of a
+ * batch of top-aligned k-mers with labels 1 (present) / 0 (absent), taken
+ * through the bf16-rounded weights the forward uses. */
+int sbwt_mlp_grad(sbwt_mlp *nn, const uint64_t *d_kmers, const float *d_labels,
+                  uint64_t batch, float *d_grad, float *d_loss, void *stream);
+/* One Adam step (beta 0.9 / 0.999) on the batch's gradient; d_loss optional. */
+int sbwt_mlp_train_step(sbwt_mlp *nn, const uint64_t *d_kmers, const float *d_labels,
+                        uint64_t batch, float lr, float *d_loss, void *stream);
+/* Logits of count k-mers on the matrix cores (reference != 0: the FP32
+ * vector kernel with the same rounding, for cross-checking). */
+int sbwt_mlp_logits(const sbwt_mlp *nn, const uint64_t *d_kmers, uint64_t count,
+                    float *d_logits, int reference, void *stream);
+
+/* Blocked Bloom filter over k-mers, sized for `expected` keys at
+ * false-positive rate fpr. Keys are top-aligned k-mers; only their top 2k
+ * bits are hashed, so 32-character windows (sbwt_extract_kmers) and k-mers
+ * with zeroed low bits (sbwt_enumerate_kmers) are the same key. */
+int sbwt_bloom_create(uint32_t k, uint64_t expected, double fpr, sbwt_bloom **out);
+int sbwt_bloom_free(sbwt_bloom *b);
+uint64_t sbwt_bloom_bytes(const sbwt_bloom *b);
+int sbwt_bloom_insert(sbwt_bloom *b, const uint64_t *d_keys, uint64_t count, void *stream);
+int sbwt_bloom_query(const sbwt_bloom *b, const uint64_t *d_keys, uint64_t count,
+                     uint8_t *d_out, void *stream);
+
+/* d_maybe[q] = 1 if logit(k-mer q) >= threshold or the Bloom filter (may be
+ * NULL; built for the model's k) holds it; k-mers with 0 are absent from the
+ * index for certain when the filter holds every indexed k-mer whose logit is
+ * below threshold. Only the top 2k bits of each key are read. */
+int sbwt_prefilter(const sbwt_mlp *nn, float threshold, const sbwt_bloom *b,
+                   const uint64_t *d_kmers, uint64_t count, uint8_t *d_maybe,
+                   void *stream);
 
 /* ---- measurement ------------------------------------------------------ */
 

@@ -1,0 +1,187 @@
+"""CPU oracle of the learned prefilter's MLP (test infrastructure only).
+
+The network (no reference counterpart): one-hot 2-bit k-mer (character t,
+code c -> input 4t + c) -> 128 ReLU -> 64 ReLU -> 1 logit; loss = mean binary
+cross-entropy with logits. The flat fp32 parameter vector is
+W1[4k][128], b1[128], W2[128][64], b2[64], w3[64], b3.
+
+Rounding rules the GPU kernels follow and this oracle restates:
+  * W1, W2, w3 are used as bf16 (round to nearest even of the fp32 master);
+    biases stay fp32;
+  * training forward keeps h1, h2 in full precision; inference rounds h1
+    (after ReLU) to bf16, because it feeds the second layer's bf16 MFMA;
+  * gradients are taken with respect to the rounded weights and applied to
+    the fp32 masters (straight-through), Adam with beta 0.9 / 0.999, eps 1e-8.
+
+Everything here is float64 numpy. Pinned in tests/test_oracle_mlp.py by
+central finite differences of the loss and by torch autograd over an
+independently written forward (dense one-hot matmul).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+H1, H2 = 128, 64
+
+
+def shape(k: int) -> dict:
+    i = 4 * k
+    o = {"in": i, "W1": 0}
+    o["b1"] = o["W1"] + i * H1
+    o["W2"] = o["b1"] + H1
+    o["b2"] = o["W2"] + H1 * H2
+    o["w3"] = o["b2"] + H2
+    o["b3"] = o["w3"] + H2
+    o["total"] = o["b3"] + 1
+    return o
+
+
+def bf16_round(x: np.ndarray) -> np.ndarray:
+    """fp32 -> bf16 -> fp32 with round-to-nearest-even (finite inputs)."""
+    f = np.ascontiguousarray(x, dtype=np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32)
+
+
+def unpack(params: np.ndarray, k: int, rounded: bool = True):
+    """Weights and biases as float64 arrays. rounded: the fp32 masters with
+    W1, W2, w3 rounded to bf16 (what the kernels compute with); otherwise the
+    values exactly as given (float64), for finite-difference checks."""
+    o = shape(k)
+    if rounded:
+        p = np.asarray(params, dtype=np.float32)
+        r = bf16_round
+    else:
+        p = np.asarray(params, dtype=np.float64)
+        r = (lambda a: a)
+    W1 = r(p[o["W1"]:o["b1"]]).astype(np.float64).reshape(4 * k, H1)
+    b1 = p[o["b1"]:o["W2"]].astype(np.float64)
+    W2 = r(p[o["W2"]:o["b2"]]).astype(np.float64).reshape(H1, H2)
+    b2 = p[o["b2"]:o["w3"]].astype(np.float64)
+    w3 = r(p[o["w3"]:o["b3"]]).astype(np.float64)
+    b3 = float(p[o["b3"]])
+    return W1, b1, W2, b2, w3, b3
+
+
+def chars(kmers: np.ndarray, k: int) -> np.ndarray:
+    """[B][k] 2-bit codes of top-aligned k-mers (character 0 in bits 63..62)."""
+    km = np.asarray(kmers, dtype=np.uint64)
+    sh = np.array([62 - 2 * t for t in range(k)], dtype=np.uint64)
+    return ((km[:, None] >> sh[None, :]) & np.uint64(3)).astype(np.int64)
+
+
+def first_layer(W1, kmers, k):
+    """sum_t W1[4t + c_t] for every k-mer: the one-hot product as a gather."""
+    c = chars(kmers, k)
+    idx = 4 * np.arange(k)[None, :] + c
+    return W1[idx].sum(axis=1)
+
+
+def forward(params, kmers, k, inference: bool = False, rounded: bool = True):
+    W1, b1, W2, b2, w3, b3 = unpack(params, k, rounded)
+    a1 = first_layer(W1, kmers, k) + b1
+    h1 = np.maximum(a1, 0.0)
+    if inference:
+        h1 = bf16_round(h1.astype(np.float32)).astype(np.float64)
+    a2 = h1 @ W2 + b2
+    h2 = np.maximum(a2, 0.0)
+    z = h2 @ w3 + b3
+    return z, (a1, h1, a2, h2)
+
+
+def logits(params, kmers, k):
+    """Inference logits (h1 rounded to bf16, as the matrix-core kernel does)."""
+    return forward(params, kmers, k, inference=True)[0]
+
+
+def loss_and_grad(params, kmers, labels, k, rounded: bool = True):
+    """Mean BCE-with-logits and its gradient (flat, same layout as params)
+    with respect to the bf16-rounded weights and fp32 biases (rounded=False:
+    with respect to the float64 parameters as given)."""
+    o = shape(k)
+    W1, b1, W2, b2, w3, b3 = unpack(params, k, rounded)
+    y = np.asarray(labels, dtype=np.float64)
+    B = y.size
+    z, (a1, h1, a2, h2) = forward(params, kmers, k, rounded=rounded)
+    loss = float(np.mean(np.logaddexp(0.0, z) - y * z))
+    dz = (1.0 / (1.0 + np.exp(-z)) - y) / B
+    g = np.zeros(o["total"], dtype=np.float64)
+    g[o["b3"]] = dz.sum()
+    g[o["w3"]:o["b3"]] = h2.T @ dz
+    dh2 = np.outer(dz, w3) * (a2 > 0)
+    g[o["b2"]:o["w3"]] = dh2.sum(axis=0)
+    g[o["W2"]:o["b2"]] = (h1.T @ dh2).reshape(-1)
+    dh1 = (dh2 @ W2.T) * (a1 > 0)
+    g[o["b1"]:o["W2"]] = dh1.sum(axis=0)
+    c = chars(kmers, k)
+    gW1 = np.zeros((4 * k, H1))
+    for t in range(k):
+        np.add.at(gW1, 4 * t + c[:, t], dh1)
+    g[o["W1"]:o["b1"]] = gW1.reshape(-1)
+    return loss, g
+
+
+def adam_step(params, m, v, grad, step: int, lr: float):
+    """One Adam update (beta1 0.9, beta2 0.999, eps 1e-8) in float64;
+    returns (params, m, v)."""
+    p = np.asarray(params, dtype=np.float64)
+    m = 0.9 * m + 0.1 * grad
+    v = 0.999 * v + 0.001 * grad * grad
+    b1c = 1.0 - 0.9 ** step
+    b2c = 1.0 - 0.999 ** step
+    return p - lr * (m / b1c) / (np.sqrt(v / b2c) + 1e-8), m, v
+
+
+def he_params(k: int, seed: int) -> np.ndarray:
+    """A random parameter vector of the right size (numpy PCG64), He scale."""
+    o = shape(k)
+    rng = np.random.default_rng(seed)
+    p = np.zeros(o["total"], dtype=np.float32)
+    p[o["W1"]:o["b1"]] = rng.normal(0, np.sqrt(2.0 / k), o["b1"] - o["W1"])
+    p[o["b1"]:o["W2"]] = rng.normal(0, 0.1, H1)
+    p[o["W2"]:o["b2"]] = rng.normal(0, np.sqrt(2.0 / H1), o["b2"] - o["W2"])
+    p[o["b2"]:o["w3"]] = rng.normal(0, 0.1, H2)
+    p[o["w3"]:o["b3"]] = rng.normal(0, np.sqrt(1.0 / H2), H2)
+    p[o["b3"]] = 0.05
+    return p
+
+
+# ---- Bloom filter (the GPU filter's hash scheme, for cross-checks) ---------
+
+M64 = (1 << 64) - 1
+
+
+def mix64(x: int) -> int:
+    """splitmix64's finaliser (Steele, Lea, Flood 2014)."""
+    x ^= x >> 30
+    x = (x * 0xBF58476D1CE4E5B9) & M64
+    x ^= x >> 27
+    x = (x * 0x94D049BB133111EB) & M64
+    x ^= x >> 31
+    return x
+
+
+def bloom_geometry(expected: int, fpr: float):
+    """(512-bit blocks, hashes): 1.5x the bits of an unblocked filter,
+    -n ln p / ln^2 2, and round(-ln p / ln 2) hashes."""
+    import math
+
+    bits = 1.5 * (-expected * math.log(fpr) / (math.log(2) ** 2)) if expected else 0.0
+    blocks = int(math.ceil(bits / 512.0)) if expected else 0
+    hashes = max(1, int(round(-math.log(fpr) / math.log(2))))
+    return blocks, hashes
+
+
+def bloom_bits(key: int, blocks: int, hashes: int):
+    """(block, [bit positions in the block]) of a key."""
+    h = mix64((key ^ 0x9E3779B97F4A7C15) & M64)
+    blk = (h * blocks) >> 64
+    h2 = mix64(h)
+    out = []
+    for i in range(hashes):
+        out.append(h2 & 511)
+        h2 = ((h2 >> 9) | (h2 << 55)) & M64
+        if i % 7 == 6:
+            h2 = mix64(h2)
+    return blk, out

@@ -6,7 +6,8 @@ torch tensors only as device buffers. Importing the package does not load
 the library; the first call into it does, and fails loudly if it is missing.
 """
 from ._capi import SbwtError, load  # noqa: F401
-from .api import NONE, Config, Index, ReadBatch, ReadsPlan, format_results, pack, parse_lines, to_u64  # noqa: F401
+from .api import (NONE, Bloom, Config, Index, Mlp, ReadBatch, ReadsPlan, format_results, pack,  # noqa: F401
+                  parse_lines, prefilter, to_u64)
 
-__all__ = ["Config", "Index", "ReadBatch", "ReadsPlan", "pack", "parse_lines", "format_results", "to_u64", "NONE", "SbwtError",
+__all__ = ["Config", "Index", "ReadBatch", "ReadsPlan", "Mlp", "Bloom", "prefilter", "pack", "parse_lines", "format_results", "to_u64", "NONE", "SbwtError",
            "load"]

@@ -55,12 +55,27 @@ SIGNATURES = {
     "sbwt_kmer_offsets": (i32, [vp, u64, u32, vp, vp]),
     "sbwt_positions": (i32, [vp, vp, u64, u32, vp, vp]),
     "sbwt_extract_kmers": (i32, [vp, vp, u64, vp, vp]),
+    "sbwt_search_reads_masked": (i32, [vp, vp, vp, vp, vp, vp, vp, u64, u64, u32, vp, vp, u32, vp]),
     "sbwt_reads_work_bound": (u64, [u64, u64, u32]),
     "sbwt_reads_work": (i32, [vp, u64, u32, vp, vp, vp]),
     "sbwt_search_reads": (i32, [vp, vp, vp, vp, vp, vp, vp, u64, u64, u32, vp, u32, vp]),
     "sbwt_select": (i32, [vp, u32, vp, u64, vp, vp]),
     "sbwt_node_labels": (i32, [vp, vp, vp, vp]),
     "sbwt_enumerate_kmers": (i32, [vp, vp, u64p, vp]),
+    "sbwt_mlp_create": (i32, [u32, u64, u64, C.POINTER(vp)]),
+    "sbwt_mlp_free": (i32, [vp]),
+    "sbwt_mlp_param_count": (u64, [vp]),
+    "sbwt_mlp_set_params": (i32, [vp, vp, u64]),
+    "sbwt_mlp_get_params": (i32, [vp, vp, u64]),
+    "sbwt_mlp_grad": (i32, [vp, vp, vp, u64, vp, vp, vp]),
+    "sbwt_mlp_train_step": (i32, [vp, vp, vp, u64, C.c_float, vp, vp]),
+    "sbwt_mlp_logits": (i32, [vp, vp, u64, vp, i32, vp]),
+    "sbwt_bloom_create": (i32, [u32, u64, C.c_double, C.POINTER(vp)]),
+    "sbwt_bloom_free": (i32, [vp]),
+    "sbwt_bloom_bytes": (u64, [vp]),
+    "sbwt_bloom_insert": (i32, [vp, vp, u64, vp]),
+    "sbwt_bloom_query": (i32, [vp, vp, u64, vp, vp]),
+    "sbwt_prefilter": (i32, [vp, C.c_float, vp, vp, u64, vp, vp]),
     "sbwt_gather_roof": (i32, [vp, u64, u32, u32, u32, u64, u64, vp, vp]),
     "sbwt_parse_lines": (i32, [cp, u64, vp, vp, u64, u64p]),
     "sbwt_format_results": (i32, [vp, vp, u64, vp, u64, u64p]),

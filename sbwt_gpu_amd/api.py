@@ -215,7 +215,9 @@ class Index:
 
     def search_reads(self, reads: "ReadBatch", out: torch.Tensor | None = None, n_as_a: bool = False,
                      independent: bool = False, count: bool = False, persistent: bool = False,
-                     certificates: bool = True) -> torch.Tensor:
+                     certificates: bool = True, maybe: torch.Tensor | None = None) -> torch.Tensor:
+        """Answers for every k-mer of the batch; `maybe` (uint8, one per k-mer,
+        e.g. from prefilter()) skips the k-mers it marks 0."""
         if reads.k != self.k:
             raise ValueError(f"reads prepared for k={reads.k}, index has k={self.k}")
         if out is None:
@@ -223,9 +225,12 @@ class Index:
         flags = ((_capi.READS_N_AS_A if n_as_a else 0) | (_capi.READS_INDEPENDENT if independent else 0)
                  | (_capi.COUNT if count else 0) | (_capi.READS_PERSISTENT if persistent else 0)
                  | (0 if certificates else _capi.READS_NO_CERT))
-        check(self.lib.sbwt_search_reads(self._h, _ptr(reads.packed), _ptr(reads.nmask), _ptr(reads.row_starts),
-                                         _ptr(reads.offsets), _ptr(reads.work), _ptr(reads.items), reads.rows,
-                                         reads.work_bound, reads.seg, _ptr(out), flags, _stream()))
+        args = (self._h, _ptr(reads.packed), _ptr(reads.nmask), _ptr(reads.row_starts), _ptr(reads.offsets),
+                _ptr(reads.work), _ptr(reads.items), reads.rows, reads.work_bound, reads.seg)
+        if maybe is None:
+            check(self.lib.sbwt_search_reads(*args, _ptr(out), flags, _stream()))
+        else:
+            check(self.lib.sbwt_search_reads_masked(*args, _ptr(maybe), _ptr(out), flags, _stream()))
         return out[: reads.total_kmers]
 
 
@@ -274,6 +279,14 @@ class ReadBatch:
         chars, starts = _upload_rows(seqs, device)
         hs = starts.cpu().numpy()
         return cls(chars, starts, k, seg, host_starts=hs)
+
+    def kmers(self) -> torch.Tensor:
+        """Top-aligned code of every k-mer window, in output order."""
+        L = _capi.load()
+        pos = self.positions()
+        out = torch.empty_like(pos)
+        check(L.sbwt_extract_kmers(_ptr(self.packed), _ptr(pos), pos.numel(), _ptr(out), _stream()))
+        return out
 
     def positions(self) -> torch.Tensor:
         """Start offset of every k-mer (include/gpu/positions_kernel.hpp:18-36)."""
@@ -364,3 +377,96 @@ class ReadsPlan:
             self.close()
         except Exception:
             pass
+
+
+class Mlp:
+    """The prefilter's MLP (4k -> 128 -> 64 -> 1), bf16 weights, FP32
+    accumulation, trained on the GPU (see include/sbwt_gpu_amd.h)."""
+
+    def __init__(self, k: int = 31, seed: int = 1, max_batch: int = 65536):
+        self.lib = _capi.load()
+        self._h = C.c_void_p()
+        check(self.lib.sbwt_mlp_create(k, seed, max_batch, C.byref(self._h)))
+        self.k, self.max_batch = k, max_batch
+        self.param_count = int(self.lib.sbwt_mlp_param_count(self._h))
+        self._loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+
+    @property
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.param_count, np.float32)
+        check(self.lib.sbwt_mlp_get_params(self._h, C.c_void_p(p.ctypes.data), p.size))
+        return p
+
+    @params.setter
+    def params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        check(self.lib.sbwt_mlp_set_params(self._h, C.c_void_p(p.ctypes.data), p.size))
+
+    def grad(self, kmers: torch.Tensor, labels: torch.Tensor):
+        g = torch.empty(self.param_count, dtype=torch.float32, device="cuda")
+        check(self.lib.sbwt_mlp_grad(self._h, _ptr(kmers), _ptr(labels), kmers.numel(), _ptr(g), _ptr(self._loss),
+                                     _stream()))
+        return g, self._loss
+
+    def train_step(self, kmers: torch.Tensor, labels: torch.Tensor, lr: float) -> torch.Tensor:
+        check(self.lib.sbwt_mlp_train_step(self._h, _ptr(kmers), _ptr(labels), kmers.numel(), C.c_float(lr),
+                                           _ptr(self._loss), _stream()))
+        return self._loss
+
+    def logits(self, kmers: torch.Tensor, reference: bool = False) -> torch.Tensor:
+        out = torch.empty(kmers.numel(), dtype=torch.float32, device="cuda")
+        check(self.lib.sbwt_mlp_logits(self._h, _ptr(kmers), kmers.numel(), _ptr(out), int(reference), _stream()))
+        return out
+
+    def close(self):
+        if self._h:
+            self.lib.sbwt_mlp_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Bloom:
+    """Blocked Bloom filter over top-aligned k-mers in HBM (only the top 2k
+    bits of a key are hashed)."""
+
+    def __init__(self, k: int, expected: int, fpr: float = 0.01):
+        self.lib = _capi.load()
+        self._h = C.c_void_p()
+        check(self.lib.sbwt_bloom_create(k, expected, fpr, C.byref(self._h)))
+        self.k = k
+        self.bytes = int(self.lib.sbwt_bloom_bytes(self._h))
+
+    def insert(self, keys: torch.Tensor) -> None:
+        check(self.lib.sbwt_bloom_insert(self._h, _ptr(keys), keys.numel(), _stream()))
+
+    def query(self, keys: torch.Tensor) -> torch.Tensor:
+        out = torch.empty(keys.numel(), dtype=torch.uint8, device="cuda")
+        check(self.lib.sbwt_bloom_query(self._h, _ptr(keys), keys.numel(), _ptr(out), _stream()))
+        return out
+
+    def close(self):
+        if self._h:
+            self.lib.sbwt_bloom_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def prefilter(mlp: Mlp, threshold: float, bloom: "Bloom | None", kmers: torch.Tensor,
+              out: torch.Tensor | None = None) -> torch.Tensor:
+    """maybe[q] = logit >= threshold or bloom hit (fused matrix-core kernel)."""
+    L = _capi.load()
+    if out is None:
+        out = torch.empty(kmers.numel(), dtype=torch.uint8, device="cuda")
+    check(L.sbwt_prefilter(mlp._h, C.c_float(threshold), bloom._h if bloom is not None else None, _ptr(kmers),
+                           kmers.numel(), _ptr(out), _stream()))
+    return out

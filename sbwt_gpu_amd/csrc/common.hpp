@@ -15,6 +15,7 @@ namespace sbwt {
 using u64 = uint64_t;
 using u32 = uint32_t;
 using u8 = uint8_t;
+using u16 = uint16_t;
 
 constexpr u64 kNone = ~0ull;
 

@@ -199,13 +199,16 @@ struct Rolling {
 // [lo-1, d] that the k-mers in between share. A dead string w proves every
 // k-mer containing w absent, because every substring of an indexed k-mer
 // ends some node label (follow the k-mer's incoming edges back to the root).
+// With `maybe` (one byte per k-mer, in output order), k-mers marked 0 by a
+// prefilter answer ~0 without a search.
 template <int CB, bool STREAM, bool NBREAK, bool CERT, bool COUNT>
 __global__ __launch_bounds__(256) void k_search_reads(DevIndex ix, const u64* __restrict__ packed,
                                                       const u64* __restrict__ nmask,
                                                       const u64* __restrict__ row_starts,
                                                       const u64* __restrict__ offsets,
                                                       const u64* __restrict__ work, u64 rows, u64 bound, u32 seg,
-                                                      u64* __restrict__ out, unsigned long long* ctr) {
+                                                      const u8* __restrict__ maybe, u64* __restrict__ out,
+                                                      unsigned long long* ctr) {
   const u64 gid = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x;
   Tally t;
   u32 done = 0;
@@ -222,6 +225,7 @@ __global__ __launch_bounds__(256) void k_search_reads(DevIndex ix, const u64* __
     const u64 cnt = (rowk - j0) < seg ? (rowk - j0) : seg;
     const u64 rs = row_starts[r];
     u64* o = out + offsets[r];
+    const u8* mb = maybe ? maybe + offsets[r] : nullptr;
     const u32 k = ix.k;
     bool prev = false;
     u32 pnode = 0;
@@ -237,6 +241,8 @@ __global__ __launch_bounds__(256) void k_search_reads(DevIndex ix, const u64* __
       if (CERT && j >= clo && j <= chi) {
         res = kNone;
       } else if (NBREAK && (bad.at(rs + j) >> (64 - k)) != 0) {
+        res = kNone;
+      } else if (mb && !mb[j]) {
         res = kNone;
       } else if (CERT && j == kpos) {
         res = knode;
@@ -301,6 +307,7 @@ struct ReadArgs {
   const u64 *packed, *nmask, *rs, *off, *work;
   u64 rows, bound;
   u32 seg;
+  const u8* maybe;
   u64* out;
 };
 
@@ -308,7 +315,7 @@ template <int CB, bool STREAM, bool NBREAK, bool CERT, bool COUNT>
 static int launch_reads5(const sbwt_index* ix, const ReadArgs& a, hipStream_t s) {
   const u32 bt = 256;
   k_search_reads<CB, STREAM, NBREAK, CERT, COUNT><<<(a.bound + bt - 1) / bt, bt, 0, s>>>(
-      dev_view(ix), a.packed, a.nmask, a.rs, a.off, a.work, a.rows, a.bound, a.seg, a.out, ix->d_counters);
+      dev_view(ix), a.packed, a.nmask, a.rs, a.off, a.work, a.rows, a.bound, a.seg, a.maybe, a.out, ix->d_counters);
   SBWT_HIP(hipGetLastError());
   return SBWT_OK;
 }
@@ -374,11 +381,10 @@ int sbwt_search_positions(const sbwt_index* ix, const uint64_t* d_packed, uint64
   }
 }
 
-int sbwt_search_reads(const sbwt_index* ix, const uint64_t* d_packed, const uint64_t* d_nmask,
-                      const uint64_t* d_row_starts, const uint64_t* d_offsets, const uint64_t* d_work,
-                      const uint32_t* d_items, uint64_t rows, uint64_t work_bound, uint32_t seg, uint64_t* d_out,
-                      uint32_t flags, void* stream) {
-  clear_error();
+static int search_reads_impl(const sbwt_index* ix, const uint64_t* d_packed, const uint64_t* d_nmask,
+                             const uint64_t* d_row_starts, const uint64_t* d_offsets, const uint64_t* d_work,
+                             const uint32_t* d_items, uint64_t rows, uint64_t work_bound, uint32_t seg,
+                             const uint8_t* d_maybe, uint64_t* d_out, uint32_t flags, void* stream) {
   SBWT_REQUIRE(ix, "index is NULL");
   if (rows == 0 || work_bound == 0) return SBWT_OK;
   SBWT_REQUIRE(d_packed && d_row_starts && d_offsets && d_work && d_out, "device buffers must not be NULL");
@@ -391,16 +397,35 @@ int sbwt_search_reads(const sbwt_index* ix, const uint64_t* d_packed, const uint
   // streamed or certified (the reference's way)
   const bool stream_mode = !independent && ix->has_groups;
   const bool cert = !independent && !(flags & SBWT_READS_NO_CERT);
-  if (flags & SBWT_READS_PERSISTENT)
+  if ((flags & SBWT_READS_PERSISTENT) && !d_maybe)
     return search_reads_v2(ix, d_packed, d_nmask, d_row_starts, d_offsets, d_work, d_items, rows, work_bound, seg,
                            d_out, nbreak, stream_mode, cert, cnt, as_stream(stream));
-  const ReadArgs a{d_packed, d_nmask, d_row_starts, d_offsets, d_work, rows, work_bound, seg, d_out};
+  const ReadArgs a{d_packed, d_nmask, d_row_starts, d_offsets, d_work, rows, work_bound, seg, d_maybe, d_out};
   hipStream_t s = as_stream(stream);
   switch (ix->layout) {
     case 16: return launch_reads<16>(ix, a, stream_mode, nbreak, cert, cnt, s);
     case 32: return launch_reads<32>(ix, a, stream_mode, nbreak, cert, cnt, s);
     default: return launch_reads<64>(ix, a, stream_mode, nbreak, cert, cnt, s);
   }
+}
+
+int sbwt_search_reads(const sbwt_index* ix, const uint64_t* d_packed, const uint64_t* d_nmask,
+                      const uint64_t* d_row_starts, const uint64_t* d_offsets, const uint64_t* d_work,
+                      const uint32_t* d_items, uint64_t rows, uint64_t work_bound, uint32_t seg, uint64_t* d_out,
+                      uint32_t flags, void* stream) {
+  clear_error();
+  return search_reads_impl(ix, d_packed, d_nmask, d_row_starts, d_offsets, d_work, d_items, rows, work_bound, seg,
+                           nullptr, d_out, flags, stream);
+}
+
+int sbwt_search_reads_masked(const sbwt_index* ix, const uint64_t* d_packed, const uint64_t* d_nmask,
+                             const uint64_t* d_row_starts, const uint64_t* d_offsets, const uint64_t* d_work,
+                             const uint32_t* d_items, uint64_t rows, uint64_t work_bound, uint32_t seg,
+                             const uint8_t* d_maybe, uint64_t* d_out, uint32_t flags, void* stream) {
+  clear_error();
+  SBWT_REQUIRE(d_maybe, "d_maybe must not be NULL");
+  return search_reads_impl(ix, d_packed, d_nmask, d_row_starts, d_offsets, d_work, d_items, rows, work_bound, seg,
+                           d_maybe, d_out, flags, stream);
 }
 
 }  // extern "C"
