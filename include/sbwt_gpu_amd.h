@@ -274,13 +274,17 @@ int sbwt_mlp_train_step(sbwt_mlp *nn, const uint64_t *d_kmers, const float *d_la
 int sbwt_mlp_logits(const sbwt_mlp *nn, const uint64_t *d_kmers, uint64_t count,
                     float *d_logits, int reference, void *stream);
 
-/* Blocked Bloom filter over k-mers, sized for `expected` keys at
- * false-positive rate fpr. Keys are top-aligned k-mers; only their top 2k
- * bits are hashed, so 32-character windows (sbwt_extract_kmers) and k-mers
- * with zeroed low bits (sbwt_enumerate_kmers) are the same key. */
+/* Register-blocked Bloom filter over k-mers (all bits of a key in one
+ * 64-bit word), sized for `expected` keys at false-positive rate fpr. Keys
+ * are top-aligned k-mers; only their top 2k bits are hashed, so
+ * 32-character windows (sbwt_extract_kmers) and k-mers with zeroed low bits
+ * (sbwt_enumerate_kmers) are the same key. */
 int sbwt_bloom_create(uint32_t k, uint64_t expected, double fpr, sbwt_bloom **out);
 int sbwt_bloom_free(sbwt_bloom *b);
 uint64_t sbwt_bloom_bytes(const sbwt_bloom *b);
+uint32_t sbwt_bloom_hashes(const sbwt_bloom *b);
+/* Host copy of the filter's words (count = sbwt_bloom_bytes / 8). */
+int sbwt_bloom_get_words(const sbwt_bloom *b, uint64_t *words, uint64_t count);
 int sbwt_bloom_insert(sbwt_bloom *b, const uint64_t *d_keys, uint64_t count, void *stream);
 int sbwt_bloom_query(const sbwt_bloom *b, const uint64_t *d_keys, uint64_t count,
                      uint8_t *d_out, void *stream);

@@ -147,9 +147,15 @@ def he_params(k: int, seed: int) -> np.ndarray:
     return p
 
 
-# ---- Bloom filter (the GPU filter's hash scheme, for cross-checks) ---------
+# ---- Bloom filter: register-blocked, 64-bit words ----------------------------
+# The GPU filter's scheme, restated: word = high 64 bits of h * words with
+# h = mix64(key ^ golden); its `hashes` bits are bits 6i..6i+5 of mix64(h).
+# Sized by bloom_size: the smallest bits/key (grid of 1/20 of the unblocked
+# -ln p / ln^2 2) and hash count 1..10 whose Poisson-averaged false-positive
+# rate reaches p (Putze, Sanders, Singler 2007, register-blocked filters).
 
 M64 = (1 << 64) - 1
+GOLDEN = 0x9E3779B97F4A7C15
 
 
 def mix64(x: int) -> int:
@@ -162,26 +168,90 @@ def mix64(x: int) -> int:
     return x
 
 
-def bloom_geometry(expected: int, fpr: float):
-    """(512-bit blocks, hashes): 1.5x the bits of an unblocked filter,
-    -n ln p / ln^2 2, and round(-ln p / ln 2) hashes."""
+def mix64_np(x: np.ndarray) -> np.ndarray:
+    x = np.asarray(x, dtype=np.uint64).copy()
+    with np.errstate(over="ignore"):
+        x ^= x >> np.uint64(30)
+        x *= np.uint64(0xBF58476D1CE4E5B9)
+        x ^= x >> np.uint64(27)
+        x *= np.uint64(0x94D049BB133111EB)
+        x ^= x >> np.uint64(31)
+    return x
+
+
+def mulhi64_np(a: np.ndarray, b: int) -> np.ndarray:
+    """High 64 bits of the 128-bit products a * b (uint64)."""
+    m32 = np.uint64(0xFFFFFFFF)
+    a = np.asarray(a, dtype=np.uint64)
+    b = np.uint64(b)
+    a_lo, a_hi = a & m32, a >> np.uint64(32)
+    b_lo, b_hi = b & m32, b >> np.uint64(32)
+    p0, p1, p2, p3 = a_lo * b_lo, a_lo * b_hi, a_hi * b_lo, a_hi * b_hi
+    mid = (p0 >> np.uint64(32)) + (p1 & m32) + (p2 & m32)
+    return p3 + (p1 >> np.uint64(32)) + (p2 >> np.uint64(32)) + (mid >> np.uint64(32))
+
+
+def bloom_fpr(bits_per_key: float, h: int) -> float:
     import math
 
-    bits = 1.5 * (-expected * math.log(fpr) / (math.log(2) ** 2)) if expected else 0.0
-    blocks = int(math.ceil(bits / 512.0)) if expected else 0
-    hashes = max(1, int(round(-math.log(fpr) / math.log(2))))
-    return blocks, hashes
+    lam = 64.0 / bits_per_key
+    per = math.log1p(-1.0 / 64.0) * h
+    pmf, s = math.exp(-lam), 0.0
+    nmax = int(lam + 12.0 * math.sqrt(lam) + 24.0)
+    for n in range(nmax + 1):
+        s += pmf * (1.0 - math.exp(per * n)) ** h
+        pmf *= lam / (n + 1)
+    return s
 
 
-def bloom_bits(key: int, blocks: int, hashes: int):
-    """(block, [bit positions in the block]) of a key."""
-    h = mix64((key ^ 0x9E3779B97F4A7C15) & M64)
-    blk = (h * blocks) >> 64
-    h2 = mix64(h)
-    out = []
+def bloom_size(p: float):
+    """(bits per key, hashes) for false-positive rate p."""
+    import math
+
+    base = -math.log(p) / (math.log(2) * math.log(2))
+    best, hashes = 0.0, 1
+    for h in range(1, 11):
+        for j in range(201):
+            bpk = base * (1.0 + 0.05 * j)
+            if bloom_fpr(bpk, h) <= p:
+                if best == 0.0 or bpk < best:
+                    best, hashes = bpk, h
+                break
+    if best == 0.0:
+        best, hashes = base * 11.0, 10
+    return best, hashes
+
+
+def bloom_geometry(expected: int, fpr: float):
+    """(64-bit words, hashes) of a filter for `expected` keys."""
+    import math
+
+    bpk, h = bloom_size(fpr)
+    return (int(math.ceil(expected * bpk / 64.0)) if expected else 0), h
+
+
+def key_mask(k: int) -> int:
+    return M64 if k == 32 else (M64 ^ ((1 << (64 - 2 * k)) - 1))
+
+
+def bloom_locate(keys: np.ndarray, words: int, hashes: int, k: int):
+    """(word index, bit mask) of every key (numpy uint64 arrays)."""
+    key = np.asarray(keys, dtype=np.uint64) & np.uint64(key_mask(k))
+    h = mix64_np(key ^ np.uint64(GOLDEN))
+    h2 = mix64_np(h)
+    mask = np.zeros_like(h)
     for i in range(hashes):
-        out.append(h2 & 511)
-        h2 = ((h2 >> 9) | (h2 << 55)) & M64
-        if i % 7 == 6:
-            h2 = mix64(h2)
-    return blk, out
+        mask |= np.uint64(1) << ((h2 >> np.uint64(6 * i)) & np.uint64(63))
+    return mulhi64_np(h, words), mask
+
+
+def bloom_build(keys: np.ndarray, words: int, hashes: int, k: int) -> np.ndarray:
+    w, m = bloom_locate(keys, words, hashes, k)
+    out = np.zeros(words, dtype=np.uint64)
+    np.bitwise_or.at(out, w.astype(np.int64), m)
+    return out
+
+
+def bloom_query(bits: np.ndarray, keys: np.ndarray, hashes: int, k: int) -> np.ndarray:
+    w, m = bloom_locate(keys, bits.size, hashes, k)
+    return (bits[w.astype(np.int64)] & m) == m

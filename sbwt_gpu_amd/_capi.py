@@ -73,6 +73,8 @@ SIGNATURES = {
     "sbwt_bloom_create": (i32, [u32, u64, C.c_double, C.POINTER(vp)]),
     "sbwt_bloom_free": (i32, [vp]),
     "sbwt_bloom_bytes": (u64, [vp]),
+    "sbwt_bloom_hashes": (u32, [vp]),
+    "sbwt_bloom_get_words": (i32, [vp, vp, u64]),
     "sbwt_bloom_insert": (i32, [vp, vp, u64, vp]),
     "sbwt_bloom_query": (i32, [vp, vp, u64, vp, vp]),
     "sbwt_prefilter": (i32, [vp, C.c_float, vp, vp, u64, vp, vp]),

@@ -439,10 +439,16 @@ class Bloom:
         self._h = C.c_void_p()
         check(self.lib.sbwt_bloom_create(k, expected, fpr, C.byref(self._h)))
         self.k = k
+        self.hashes = int(self.lib.sbwt_bloom_hashes(self._h))
         self.bytes = int(self.lib.sbwt_bloom_bytes(self._h))
 
     def insert(self, keys: torch.Tensor) -> None:
         check(self.lib.sbwt_bloom_insert(self._h, _ptr(keys), keys.numel(), _stream()))
+
+    def words(self) -> np.ndarray:
+        w = np.zeros(self.bytes // 8, np.uint64)
+        check(self.lib.sbwt_bloom_get_words(self._h, C.c_void_p(w.ctypes.data), w.size))
+        return w
 
     def query(self, keys: torch.Tensor) -> torch.Tensor:
         out = torch.empty(keys.numel(), dtype=torch.uint8, device="cuda")

@@ -36,12 +36,17 @@ CFLAGS = [
     "-I", os.path.join(os.path.dirname(HERE), "include"),
 ]
 
+# Per-file extras. mlp.hip: matrix-core accumulators in VGPRs (the vector
+# ALU reads them every layer; AGPR accumulators would cost one
+# v_accvgpr_read per element).
+EXTRA = {"mlp.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+
 
 def _compile(src: str) -> str:
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
     deps = [src] + glob.glob(os.path.join(CSRC, "*.hpp")) + [
         os.path.join(os.path.dirname(HERE), "include", "sbwt_gpu_amd.h")]
-    cmd = [HIPCC, *CFLAGS, "-c", src, "-o", obj]
+    cmd = [HIPCC, *CFLAGS, *EXTRA.get(os.path.basename(src), []), "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{r.stderr[-6000:]}")

@@ -81,7 +81,7 @@ struct sbwt_mlp {
 };
 
 struct sbwt_bloom {
-  sbwt::u64 blocks = 0;  // 512-bit blocks
+  sbwt::u64 blocks = 0;  // 64-bit words, one per key's block
   sbwt::u32 hashes = 0;
   sbwt::u32 k = 32;
   sbwt::u64 keymask = ~0ull;  // the top 2k bits
@@ -136,18 +136,26 @@ __device__ __forceinline__ f4v mfma32(u32x4 a, u32x4 b, f4v c) {
 struct MlpRegs {
   u32x4 a1[8][4];  // [hidden block nb][input block kb]
   u32x4 a2[4][4];  // [output block mb][hidden block j]
+  f4v b1[8];       // b1 of the rows this lane's layer-1 accumulators hold
+  f4v b2[4];       // b2 and bf16-rounded w3 of its layer-2 rows
+  f4v w3[4];
+  float b3;
 };
 
-constexpr u32 kBias = kH1 + 2 * kH2 + 1;  // b1, b2, bf16-rounded w3, b3
-
-__device__ __forceinline__ void mlp_load(const MlpDev& md, MlpRegs& w, float* sb) {
-  for (u32 i = threadIdx.x; i < kH1; i += blockDim.x) sb[i] = md.p[md.ob1 + i];
-  for (u32 i = threadIdx.x; i < kH2; i += blockDim.x) {
-    sb[kH1 + i] = md.p[md.ob2 + i];
-    sb[kH1 + kH2 + i] = from_bf16(to_bf16(md.p[md.ow3 + i]));
-  }
-  if (threadIdx.x == 0) sb[kH1 + 2 * kH2] = md.p[md.ob3];
+__device__ __forceinline__ void mlp_load(const MlpDev& md, MlpRegs& w) {
   const u32 lane = threadIdx.x & 63, c = lane & 15, q = lane >> 4;
+#pragma unroll
+  for (u32 nb = 0; nb < 8; nb++)
+#pragma unroll
+    for (u32 i = 0; i < 4; i++) w.b1[nb][i] = md.p[md.ob1 + nb * 16 + q * 4 + i];
+#pragma unroll
+  for (u32 mb = 0; mb < 4; mb++)
+#pragma unroll
+    for (u32 i = 0; i < 4; i++) {
+      w.b2[mb][i] = md.p[md.ob2 + mb * 16 + q * 4 + i];
+      w.w3[mb][i] = from_bf16(to_bf16(md.p[md.ow3 + mb * 16 + q * 4 + i]));
+    }
+  w.b3 = md.p[md.ob3];
 #pragma unroll
   for (u32 nb = 0; nb < 8; nb++)
 #pragma unroll
@@ -161,88 +169,163 @@ __device__ __forceinline__ void mlp_load(const MlpDev& md, MlpRegs& w, float* sb
       const uint2 lo = *reinterpret_cast<const uint2*>(r), hi = *reinterpret_cast<const uint2*>(r + 16);
       w.a2[mb][j] = u32x4{lo.x, lo.y, hi.x, hi.y};
     }
-  __syncthreads();
-}
-
-__device__ __forceinline__ u32 pack_bf16(float lo, float hi) {
-  return static_cast<u32>(to_bf16(lo > 0.f ? lo : 0.f)) | (static_cast<u32>(to_bf16(hi > 0.f ? hi : 0.f)) << 16);
-}
-
-// Logit of k-mer `km` of this lane's column c (the 16 k-mers of a group sit
-// in the columns); every lane of column c returns it.
-__device__ __forceinline__ float mlp16(const MlpRegs& w, const float* __restrict__ sb, u64 km, u32 k) {
-  const u32 q = (threadIdx.x & 63) >> 4;
-  f4v acc1[8];
+  // The weights feed only the matrix cores, which read AGPRs directly: pin
+  // them there so the accumulators, which the vector ALU reads, get VGPRs.
 #pragma unroll
-  for (int nb = 0; nb < 8; nb++) acc1[nb] = f4v{0.f, 0.f, 0.f, 0.f};
+  for (u32 nb = 0; nb < 8; nb++)
+#pragma unroll
+    for (u32 kb = 0; kb < 4; kb++) asm volatile("" : "+a"(w.a1[nb][kb]));
+#pragma unroll
+  for (u32 mb = 0; mb < 4; mb++)
+#pragma unroll
+    for (u32 j = 0; j < 4; j++) asm volatile("" : "+a"(w.a2[mb][j]));
+}
+
+typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+typedef short s2v __attribute__((ext_vector_type(2)));
+
+// Round two values to nearest even bf16 (one v_cvt_pk_bf16_f32), then ReLU
+// both: a bf16 (or fp32) bit pattern is negative as a signed integer exactly
+// when the value is negative, so max(bits, 0) is ReLU (one v_pk_max_i16; and
+// rounding then clamping equals clamping then rounding).
+__device__ __forceinline__ u32 pack_relu_bf16(float lo, float hi) {
+  typedef float f2v __attribute__((ext_vector_type(2)));
+  const bf2v v = __builtin_convertvector(f2v{lo, hi}, bf2v);
+  const s2v r = __builtin_elementwise_max(__builtin_bit_cast(s2v, v), s2v{0, 0});
+  return __builtin_bit_cast(u32, r);
+}
+
+__device__ __forceinline__ float relu_bits(float x) {
+  return __int_as_float(max(__float_as_int(x), 0));
+}
+
+// One-hot B fragments of layer 1 by the 4-bit code of two characters
+// (c0 c1): slots 0..3 one-hot of c0, slots 4..7 one-hot of c1, bf16 1.0 at
+// the code. 16 entries of 16 bytes in LDS span the 64 banks once, so a
+// wave's lookups never conflict. Characters at or past k need no masking:
+// their columns of W1^T are zero.
+struct OneHotLds {
+  u32x4 e[16];
+};
+
+__device__ __forceinline__ void onehot_init(OneHotLds& t) {
+  if (threadIdx.x < 16) {
+    const u32 c0 = threadIdx.x >> 2, c1 = threadIdx.x & 3;
+    const u64 o0 = 0x3F80ull << (16 * c0), o1 = 0x3F80ull << (16 * c1);
+    t.e[threadIdx.x] =
+        u32x4{static_cast<u32>(o0), static_cast<u32>(o0 >> 32), static_cast<u32>(o1), static_cast<u32>(o1 >> 32)};
+  }
+}
+
+// Input block kb of k-mer km for lane quad q: characters 8kb + 2q, 8kb + 2q + 1
+// are the 4 bits at 60 - 16kb - 4q.
+__device__ __forceinline__ u32x4 onehot_block(const OneHotLds& t, u64 km, u32 kb, u32 q) {
+  const u32 word = kb < 2 ? static_cast<u32>(km >> 32) : static_cast<u32>(km);
+  const u32 code = (word >> (28 - 16 * (kb & 1) - 4 * q)) & 15u;
+  return t.e[code];
+}
+
+// Global index of this wave, as a scalar (the compiler cannot prove
+// threadIdx.x >> 6 uniform by itself).
+__device__ __forceinline__ u64 wave_index() {
+  const u32 w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  return static_cast<u64>(blockIdx.x) * (blockDim.x >> 6) + w;
+}
+
+// Logit of this lane's own k-mer: the wave's 64 k-mers as four groups of 16
+// (group g = lanes 16g .. 16g+15, one k-mer per column). Software-pipelined:
+// the layer-1 MFMAs of group g+1 are issued between the ReLU/bf16 packing and
+// layer-2 MFMAs of group g, so the matrix pipeline never drains between
+// layers; the epilogue of group g (layer 3, a dot product, on the vector
+// ALU) overlaps the next group's matrix work.
+__device__ __forceinline__ float mlp64(const MlpRegs& w, const OneHotLds& oh, u64 km) {
+  const u32 lane = threadIdx.x & 63, c = lane & 15, q = lane >> 4;
+  u64 kg[4];
+#pragma unroll
+  for (u32 g = 0; g < 4; g++) kg[g] = __shfl(km, static_cast<int>(g * 16 + c));
+  f4v a1[2][8];
+#pragma unroll
+  for (int nb = 0; nb < 8; nb++) a1[0][nb] = w.b1[nb];  // biases enter as the accumulators' start
 #pragma unroll
   for (u32 kb = 0; kb < 4; kb++) {
-    const u32 t0 = kb * 8 + q * 2, t1 = t0 + 1;
-    const u64 o0 = t0 < k ? 0x3F80ull << (16 * ((km >> (62 - 2 * t0)) & 3u)) : 0;  // bf16 1.0 at the code
-    const u64 o1 = t1 < k ? 0x3F80ull << (16 * ((km >> (62 - 2 * t1)) & 3u)) : 0;
-    const u32x4 b = u32x4{static_cast<u32>(o0), static_cast<u32>(o0 >> 32), static_cast<u32>(o1),
-                          static_cast<u32>(o1 >> 32)};
+    const u32x4 b = onehot_block(oh, kg[0], kb, q);
 #pragma unroll
-    for (u32 nb = 0; nb < 8; nb++) acc1[nb] = mfma32(w.a1[nb][kb], b, acc1[nb]);
+    for (u32 nb = 0; nb < 8; nb++) a1[0][nb] = mfma32(w.a1[nb][kb], b, a1[0][nb]);
   }
-  f4v acc2[4];
+  float zp[4];
 #pragma unroll
-  for (int mb = 0; mb < 4; mb++) acc2[mb] = f4v{0.f, 0.f, 0.f, 0.f};
+  for (u32 g = 0; g < 4; g++) {
+    const u32 cur = g & 1, nxt = cur ^ 1;
+    if (g + 1 < 4) {
 #pragma unroll
-  for (u32 j = 0; j < 4; j++) {
-    const float4 c0 = *reinterpret_cast<const float4*>(sb + (2 * j) * 16 + q * 4);
-    const float4 c1 = *reinterpret_cast<const float4*>(sb + (2 * j + 1) * 16 + q * 4);
-    const f4v x0 = acc1[2 * j], x1 = acc1[2 * j + 1];
-    const u32x4 h = u32x4{pack_bf16(x0[0] + c0.x, x0[1] + c0.y), pack_bf16(x0[2] + c0.z, x0[3] + c0.w),
-                          pack_bf16(x1[0] + c1.x, x1[1] + c1.y), pack_bf16(x1[2] + c1.z, x1[3] + c1.w)};
+      for (int nb = 0; nb < 8; nb++) a1[nxt][nb] = w.b1[nb];
+    }
+    f4v acc2[4];
 #pragma unroll
-    for (u32 mb = 0; mb < 4; mb++) acc2[mb] = mfma32(w.a2[mb][j], h, acc2[mb]);
+    for (int mb = 0; mb < 4; mb++) acc2[mb] = w.b2[mb];
+#pragma unroll
+    for (u32 j = 0; j < 4; j++) {
+      if (g + 1 < 4) {
+        const u32x4 b = onehot_block(oh, kg[g + 1], j, q);
+#pragma unroll
+        for (u32 nb = 0; nb < 8; nb++) a1[nxt][nb] = mfma32(w.a1[nb][j], b, a1[nxt][nb]);
+      }
+      const f4v x0 = a1[cur][2 * j], x1 = a1[cur][2 * j + 1];
+      const u32x4 h = u32x4{pack_relu_bf16(x0[0], x0[1]), pack_relu_bf16(x0[2], x0[3]),
+                            pack_relu_bf16(x1[0], x1[1]), pack_relu_bf16(x1[2], x1[3])};
+#pragma unroll
+      for (u32 mb = 0; mb < 4; mb++) acc2[mb] = mfma32(w.a2[mb][j], h, acc2[mb]);
+    }
+    float z = 0.f;
+#pragma unroll
+    for (u32 mb = 0; mb < 4; mb++)
+#pragma unroll
+      for (u32 i = 0; i < 4; i++) z = fmaf(relu_bits(acc2[mb][i]), w.w3[mb][i], z);
+    zp[g] = z;
   }
-  float z = 0.f;
-#pragma unroll
-  for (u32 mb = 0; mb < 4; mb++) {
-    const float4 b2 = *reinterpret_cast<const float4*>(sb + kH1 + mb * 16 + q * 4);
-    const float4 w3 = *reinterpret_cast<const float4*>(sb + kH1 + kH2 + mb * 16 + q * 4);
-    const f4v y = acc2[mb];
-    const float y0 = y[0] + b2.x, y1 = y[1] + b2.y, y2 = y[2] + b2.z, y3 = y[3] + b2.w;
-    z += (y0 > 0.f ? y0 : 0.f) * w3.x + (y1 > 0.f ? y1 : 0.f) * w3.y + (y2 > 0.f ? y2 : 0.f) * w3.z +
-         (y3 > 0.f ? y3 : 0.f) * w3.w;
-  }
-  z += __shfl_xor(z, 16);
-  z += __shfl_xor(z, 32);
-  return z + sb[kH1 + 2 * kH2];
-}
-
-// Logit of this lane's own k-mer: the wave's 64 k-mers as four groups of 16.
-__device__ __forceinline__ float mlp64(const MlpRegs& w, const float* __restrict__ sb, u64 km, u32 k) {
-  const u32 lane = threadIdx.x & 63, c = lane & 15, q = lane >> 4;
   float mine = 0.f;
 #pragma unroll
   for (u32 g = 0; g < 4; g++) {
-    const u64 kg = __shfl(km, static_cast<int>(g * 16 + c));
-    const float z = mlp16(w, sb, kg, k);
+    float z = zp[g];
+    z += __shfl_xor(z, 16);
+    z += __shfl_xor(z, 32);
     mine = q == g ? z : mine;
   }
-  return mine;
+  return mine + w.b3;
 }
 
 __global__ __launch_bounds__(256) void k_mlp_logits(MlpDev md, const u64* __restrict__ kmers, u64 count,
                                                     float* __restrict__ out) {
-  __shared__ __align__(16) float sb[kBias + 3];
+  __shared__ OneHotLds oh;
+  onehot_init(oh);
   MlpRegs w;
-  mlp_load(md, w, sb);
+  mlp_load(md, w);
+  __syncthreads();
   const u32 lane = threadIdx.x & 63;
-  const u64 wave = (blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x) >> 6;
+  const u64 wave = wave_index();  // wave-uniform (scalar): scalar loop control
   const u64 waves = (gridDim.x * static_cast<u64>(blockDim.x)) >> 6;
   const u64 groups = (count + 63) / 64;
+  auto load = [&](u64 g) -> u64 {  // unconditional, clamped: lanes past the end store nothing
+    const u64 i = g * 64 + lane;
+    return kmers[i < count ? i : count - 1];
+  };
+  // Full groups, unrolled by two with the k-mer slots swapping roles and no
+  // exit between a load and its use: each load lands in a register whose
+  // old value is dead and is in flight while the other slot computes;
+  // stores are unconditional, so each wait is exact.
+  const u64 full = count / 64;
   u64 g = wave;
-  u64 km = g < groups && g * 64 + lane < count ? kmers[g * 64 + lane] : 0;
-  for (; g < groups; g += waves) {
-    const u64 gn = g + waves, qn = gn * 64 + lane;
-    const u64 kn = gn < groups && qn < count ? kmers[qn] : 0;  // next group's k-mers, in flight meanwhile
-    const float z = mlp64(w, sb, km, md.k);
-    if (g * 64 + lane < count) out[g * 64 + lane] = z;
-    km = kn;
+  u64 ka = load(g), kb = 0;
+  for (; g + waves < full; g += 2 * waves) {
+    kb = load(g + waves);
+    out[g * 64 + lane] = mlp64(w, oh, ka);
+    ka = load(g + 2 * waves);
+    out[(g + waves) * 64 + lane] = mlp64(w, oh, kb);
+  }
+  if (g < full) out[g * 64 + lane] = mlp64(w, oh, ka);
+  if (full < groups && wave == full % waves) {  // the last, partial group
+    const float z = mlp64(w, oh, load(full));
+    if (full * 64 + lane < count) out[full * 64 + lane] = z;
   }
 }
 
@@ -471,9 +554,12 @@ static int mlp_grad(sbwt_mlp* nn, const u64* kmers, const float* y, u64 B, hipSt
   return SBWT_OK;
 }
 
-// ---- Bloom filter: 512-bit blocks, `hashes` bits per key in one block -------
+// ---- Bloom filter: register-blocked (Putze, Sanders, Singler 2007) --------
+// All `hashes` bits of a key fall in one 64-bit word: a lookup is one
+// 8-byte load and a mask compare. Word = fastrange(h, words) of
+// h = mix64(key ^ golden); bit i = bits 6i..6i+5 of mix64(h) (hashes <= 10).
 
-__device__ __forceinline__ u64 mix64(u64 x) {
+__device__ __host__ __forceinline__ u64 mix64(u64 x) {
   x ^= x >> 30;
   x *= 0xbf58476d1ce4e5b9ull;
   x ^= x >> 27;
@@ -482,55 +568,27 @@ __device__ __forceinline__ u64 mix64(u64 x) {
   return x;
 }
 
-__global__ void k_bloom_insert(u64* __restrict__ bits, u64 blocks, u32 hashes, u64 keymask,
+__device__ __forceinline__ u64 bloom_locate(u64 words, u32 hashes, u64 key, u64& mask) {
+  const u64 h = mix64(key ^ 0x9E3779B97F4A7C15ull);
+  u64 h2 = mix64(h);
+  mask = 0;
+  for (u32 i = 0; i < hashes; i++, h2 >>= 6) mask |= 1ull << (h2 & 63);
+  return static_cast<u64>((static_cast<unsigned __int128>(h) * words) >> 64);
+}
+
+__global__ void k_bloom_insert(u64* __restrict__ bits, u64 words, u32 hashes, u64 keymask,
                                const u64* __restrict__ keys, u64 count) {
   const u64 q = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x;
   if (q >= count) return;
-  const u64 h = mix64((keys[q] & keymask) ^ 0x9E3779B97F4A7C15ull);
-  const u64 blk = static_cast<u64>((static_cast<unsigned __int128>(h) * blocks) >> 64);
-  u64 h2 = mix64(h);
-  for (u32 i = 0; i < hashes; i++) {
-    const u32 b = static_cast<u32>(h2 & 511);
-    h2 = (h2 >> 9) | (h2 << 55);
-    if (i % 7 == 6) h2 = mix64(h2);
-    atomicOr(reinterpret_cast<unsigned long long*>(bits + blk * 8 + (b >> 6)), 1ull << (b & 63));
-  }
+  u64 mask;
+  const u64 w = bloom_locate(words, hashes, keys[q] & keymask, mask);
+  atomicOr(reinterpret_cast<unsigned long long*>(bits + w), mask);
 }
 
-// The block of a key (already masked to the filter's top 2k bits) and the
-// hash state its bit positions come from.
-__device__ __forceinline__ u64 bloom_block(u64 blocks, u64 key, u64& h2) {
-  const u64 h = mix64(key ^ 0x9E3779B97F4A7C15ull);
-  h2 = mix64(h);
-  return static_cast<u64>((static_cast<unsigned __int128>(h) * blocks) >> 64);
-}
-
-__device__ __forceinline__ bool bloom_test(const uint4 (&v)[4], u32 hashes, u64 h2) {
-  u64 w[8];
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    w[2 * i] = static_cast<u64>(v[i].x) | (static_cast<u64>(v[i].y) << 32);
-    w[2 * i + 1] = static_cast<u64>(v[i].z) | (static_cast<u64>(v[i].w) << 32);
-  }
-  bool all = true;
-  for (u32 i = 0; i < hashes; i++) {
-    const u32 b = static_cast<u32>(h2 & 511);
-    h2 = (h2 >> 9) | (h2 << 55);
-    if (i % 7 == 6) h2 = mix64(h2);
-    u64 word = w[0];
-#pragma unroll
-    for (int j = 1; j < 8; j++)
-      if ((b >> 6) == static_cast<u32>(j)) word = w[j];
-    all = all && ((word >> (b & 63)) & 1ull);
-  }
-  return all;
-}
-
-__device__ __forceinline__ bool bloom_has(const u64* __restrict__ bits, u64 blocks, u32 hashes, u64 key) {
-  u64 h2;
-  const uint4* p = reinterpret_cast<const uint4*>(bits + bloom_block(blocks, key, h2) * 8);
-  const uint4 v[4] = {p[0], p[1], p[2], p[3]};
-  return bloom_test(v, hashes, h2);
+__device__ __forceinline__ bool bloom_has(const u64* __restrict__ bits, u64 words, u32 hashes, u64 key) {
+  u64 mask;
+  const u64 w = bloom_locate(words, hashes, key, mask);
+  return (bits[w] & mask) == mask;
 }
 
 __global__ void k_bloom_query(const u64* __restrict__ bits, u64 blocks, u32 hashes, u64 keymask,
@@ -543,41 +601,115 @@ __global__ void k_bloom_query(const u64* __restrict__ bits, u64 blocks, u32 hash
 // maybe[q] = logit >= threshold || bloom(k-mer q); fused MLP + Bloom. Each
 // wave takes 64 k-mers per step; the Bloom blocks of the k-mers the model
 // rejects are fetched while the matrix cores run the next 64, and the k-mers
-// two steps ahead are loaded meanwhile.
+// two steps ahead are loaded meanwhile. Every load is issued by every lane
+// in straight-line code (lanes with nothing to fetch read block 0, k-mer
+// indexes are clamped), so the wait before each use covers exactly the loads
+// it needs and never the ones still in flight.
+template <bool BLOOM>
 __global__ __launch_bounds__(256) void k_prefilter(MlpDev md, float threshold, const u64* __restrict__ bits,
                                                    u64 blocks, u32 hashes, u64 keymask,
                                                    const u64* __restrict__ kmers, u64 count,
                                                    u8* __restrict__ maybe) {
-  __shared__ __align__(16) float sb[kBias + 3];
+  __shared__ OneHotLds oh;
+  onehot_init(oh);
   MlpRegs w;
-  mlp_load(md, w, sb);
+  mlp_load(md, w);
+  __syncthreads();
   const u32 lane = threadIdx.x & 63;
-  const u64 wave = (blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x) >> 6;
+  const u64 wave = wave_index();  // wave-uniform (scalar): scalar loop control
   const u64 waves = (gridDim.x * static_cast<u64>(blockDim.x)) >> 6;
   const u64 groups = (count + 63) / 64;
-  auto load = [&](u64 g) -> u64 { return g < groups && g * 64 + lane < count ? kmers[g * 64 + lane] : 0; };
+  // lanes past the end take the last k-mer (their results are not stored);
+  // no select on the loaded value, which would wait for it at once
+  auto load = [&](u64 g) -> u64 {
+    const u64 i = g * 64 + lane;
+    return kmers[i < count ? i : count - 1];
+  };
+  // Fetch the Bloom word of k-mer kc if the model rejects it (every lane
+  // loads: lanes with nothing to fetch read word 0).
+  auto fetch = [&](u64 kc, float zc, u64& v, u64& mask) {
+    const u64 wi = bloom_locate(blocks, hashes, kc & keymask, mask);
+    v = bits[zc >= threshold ? 0 : wi];
+  };
+  auto decide = [&](float zc, u64 v, u64 mask) -> u8 {
+    return zc >= threshold || (BLOOM && (v & mask) == mask) ? 1 : 0;
+  };
+  // Full groups (all 64 lanes valid): one step decides group g (k-mers kc,
+  // logits zc) while the matrix cores take group g + waves (kn), and kc's
+  // register receives group g + 2 waves. Unrolled by two with the slots
+  // swapping roles and no exit between a load and its use (so no register
+  // copy of a load in flight is needed and the compiler cannot sink a load
+  // to its use); loads, matrix work and stores are unconditional (loads
+  // clamp, surplus logits are dropped), so each wait covers exactly what it
+  // needs.
+  const u64 full = count / 64;
+  auto step = [&](u64& kc, float& zc, u64 kn, float& zn, u64 g) {
+    u64 v = 0, mask = 0;
+    if (BLOOM) fetch(kc, zc, v, mask);
+    const float zcur = zc;
+    kc = load(g + 2 * waves);
+    zn = mlp64(w, oh, kn);
+    maybe[g * 64 + lane] = decide(zcur, v, mask);
+  };
   u64 g = wave;
-  u64 km = load(g), kn = load(g + waves);
-  float z = g < groups ? mlp64(w, sb, km, md.k) : 0.f;
-  for (; g < groups; g += waves) {
-    const u64 idx = g * 64 + lane;
-    const bool need = blocks != 0 && idx < count && !(z >= threshold);
-    uint4 v[4] = {};
-    u64 h2 = 0;
-    if (need) {
-      const uint4* p = reinterpret_cast<const uint4*>(bits + bloom_block(blocks, km & keymask, h2) * 8);
-      v[0] = p[0];
-      v[1] = p[1];
-      v[2] = p[2];
-      v[3] = p[3];
+  u64 ka = load(g), kb = load(g + waves);
+  float za = mlp64(w, oh, ka), zb = 0.f;
+  for (; g + waves < full; g += 2 * waves) {
+    step(ka, za, kb, zb, g);
+    step(kb, zb, ka, za, g + waves);
+  }
+  if (g < full) {  // one full group left: its logits are in za
+    u64 v = 0, mask = 0;
+    if (BLOOM) fetch(ka, za, v, mask);
+    maybe[g * 64 + lane] = decide(za, v, mask);
+  }
+  // the last, partial group
+  if (full < groups && wave == full % waves) {
+    const u64 idx = full * 64 + lane;
+    const u64 kc = load(full);
+    const float zc = mlp64(w, oh, kc);
+    u64 v = 0, mask = 0;
+    if (BLOOM) fetch(kc, zc, v, mask);
+    const u8 m = decide(zc, v, mask);
+    if (idx < count) maybe[idx] = m;
+  }
+}
+
+// False-positive rate of a register-blocked filter (64-bit words) at
+// bits_per_key with h hashes: the load of a word is Poisson(64 / bits_per_key)
+// keys, and a non-key is accepted when all h of its bits are set.
+static double bloom_fpr(double bits_per_key, u32 h) {
+  const double lam = 64.0 / bits_per_key;
+  const double per = std::log1p(-1.0 / 64.0) * h;  // ln P(a given bit unset by one key)
+  double pmf = std::exp(-lam), s = 0.0;
+  const int nmax = static_cast<int>(lam + 12.0 * std::sqrt(lam) + 24.0);
+  for (int n = 0; n <= nmax; n++) {
+    s += pmf * std::pow(1.0 - std::exp(per * n), h);
+    pmf *= lam / (n + 1);
+  }
+  return s;
+}
+
+// The smallest bits per key (on a grid of 1/20 of the unblocked filter's
+// -ln p / ln^2 2) and the hash count (1..10) that reach false-positive rate p.
+static void bloom_size(double p, double& bits_per_key, u32& hashes) {
+  const double base = -std::log(p) / (M_LN2 * M_LN2);
+  bits_per_key = 0.0;
+  hashes = 1;
+  for (u32 h = 1; h <= 10; h++)
+    for (int j = 0; j <= 200; j++) {
+      const double bpk = base * (1.0 + 0.05 * j);
+      if (bloom_fpr(bpk, h) <= p) {
+        if (bits_per_key == 0.0 || bpk < bits_per_key) {
+          bits_per_key = bpk;
+          hashes = h;
+        }
+        break;
+      }
     }
-    const u64 knn = load(g + 2 * waves);
-    const float zn = g + waves < groups ? mlp64(w, sb, kn, md.k) : 0.f;
-    const bool m = z >= threshold || (need && bloom_test(v, hashes, h2));
-    if (idx < count) maybe[idx] = m ? 1 : 0;
-    km = kn;
-    kn = knn;
-    z = zn;
+  if (bits_per_key == 0.0) {  // p beyond reach of 10 hashes in 64-bit words
+    bits_per_key = base * 11.0;
+    hashes = 10;
   }
 }
 
@@ -754,14 +886,11 @@ int sbwt_bloom_create(uint32_t k, uint64_t expected, double fpr, sbwt_bloom** ou
   auto* b = new sbwt_bloom();
   b->k = k;
   b->keymask = k == 32 ? ~0ull : ~((1ull << (64 - 2 * k)) - 1);
-  // a standard Bloom filter needs -n ln p / ln^2 2 bits; blocking costs some
-  // accuracy, recovered with 1.5x the bits
-  const double bits = expected ? 1.5 * (-static_cast<double>(expected) * std::log(fpr) / (M_LN2 * M_LN2)) : 0.0;
-  b->blocks = expected ? static_cast<u64>(std::ceil(bits / 512.0)) : 0;
-  b->hashes = static_cast<u32>(std::lround(-std::log(fpr) / M_LN2));
-  if (b->hashes < 1) b->hashes = 1;
+  double bits_per_key = 0.0;
+  bloom_size(fpr, bits_per_key, b->hashes);
+  b->blocks = expected ? static_cast<u64>(std::ceil(static_cast<double>(expected) * bits_per_key / 64.0)) : 0;
   if (b->blocks) {
-    if (hipMalloc(&b->d_bits, b->blocks * 64) != hipSuccess || hipMemset(b->d_bits, 0, b->blocks * 64) != hipSuccess) {
+    if (hipMalloc(&b->d_bits, b->blocks * 8) != hipSuccess || hipMemset(b->d_bits, 0, b->blocks * 8) != hipSuccess) {
       delete b;
       return fail(SBWT_ERR_HIP, "out of device memory for a Bloom filter of %llu blocks", (unsigned long long)b->blocks);
     }
@@ -777,7 +906,17 @@ int sbwt_bloom_free(sbwt_bloom* b) {
   return SBWT_OK;
 }
 
-uint64_t sbwt_bloom_bytes(const sbwt_bloom* b) { return b ? b->blocks * 64 : 0; }
+uint64_t sbwt_bloom_bytes(const sbwt_bloom* b) { return b ? b->blocks * 8 : 0; }
+
+uint32_t sbwt_bloom_hashes(const sbwt_bloom* b) { return b ? b->hashes : 0; }
+
+int sbwt_bloom_get_words(const sbwt_bloom* b, uint64_t* words, uint64_t count) {
+  clear_error();
+  SBWT_REQUIRE(b && words && count == b->blocks, "expected %llu words",
+               (unsigned long long)(b ? b->blocks : 0));
+  if (count) SBWT_HIP(hipMemcpy(words, b->d_bits, count * 8, hipMemcpyDeviceToHost));
+  return SBWT_OK;
+}
 
 int sbwt_bloom_insert(sbwt_bloom* b, const uint64_t* d_keys, uint64_t count, void* stream) {
   clear_error();
@@ -809,9 +948,12 @@ int sbwt_prefilter(const sbwt_mlp* nn, float threshold, const sbwt_bloom* b, con
   SBWT_REQUIRE(!b || b->k == nn->shape.k, "the Bloom filter is for k = %u, the model for k = %u", b ? b->k : 0,
                nn->shape.k);
   const u64 groups = (count + 63) / 64;
-  k_prefilter<<<resident_grid(k_prefilter, groups), 256, 0, as_stream(stream)>>>(
-      mlp_dev(nn), threshold, b ? b->d_bits : nullptr, b ? b->blocks : 0, b ? b->hashes : 0,
-      b ? b->keymask : ~0ull, d_kmers, count, d_maybe);
+  if (b && b->blocks)
+    k_prefilter<true><<<resident_grid(k_prefilter<true>, groups), 256, 0, as_stream(stream)>>>(
+        mlp_dev(nn), threshold, b->d_bits, b->blocks, b->hashes, b->keymask, d_kmers, count, d_maybe);
+  else
+    k_prefilter<false><<<resident_grid(k_prefilter<false>, groups), 256, 0, as_stream(stream)>>>(
+        mlp_dev(nn), threshold, nullptr, 0, 0, ~0ull, d_kmers, count, d_maybe);
   SBWT_HIP(hipGetLastError());
   return SBWT_OK;
 }

@@ -149,17 +149,24 @@ def test_mlp_training_lowers_loss():
     nn.close()
 
 
-def test_bloom_no_false_negatives_and_fpr():
+@pytest.mark.parametrize("k,p", [(32, 0.01), (31, 0.001), (20, 0.05)])
+def test_bloom_matches_oracle_bit_for_bit(k, p):
+    """Geometry, every word of the filter and every query answer equal the
+    numpy restatement; no false negatives; the false-positive rate is the
+    one the sizing promises."""
     rng = np.random.default_rng(4)
-    keys = rng.integers(0, 1 << 63, 500_000, dtype=np.uint64) << np.uint64(1)
-    other = (rng.integers(0, 1 << 63, 500_000, dtype=np.uint64) << np.uint64(1)) | np.uint64(1)
-    bl = sg.Bloom(32, keys.size, 0.01)
-    blocks, hashes = om.bloom_geometry(keys.size, 0.01)
-    assert bl.bytes == blocks * 64
+    keys = rng.integers(0, 1 << 63, 300_000, dtype=np.uint64) << np.uint64(1)
+    other = rng.integers(0, 1 << 63, 300_000, dtype=np.uint64) << np.uint64(1)
+    bl = sg.Bloom(k, keys.size, p)
+    words, hashes = om.bloom_geometry(keys.size, p)
+    assert bl.bytes == words * 8 and bl.hashes == hashes
     bl.insert(dev_u64(keys))
+    want = om.bloom_build(keys, words, hashes, k)
+    np.testing.assert_array_equal(bl.words(), want)
     assert bool((bl.query(dev_u64(keys)) == 1).all().item())
-    fpr = bl.query(dev_u64(other)).float().mean().item()
-    assert 0.001 < fpr < 0.02, fpr
+    got = bl.query(dev_u64(other)).cpu().numpy().astype(bool)
+    np.testing.assert_array_equal(got, om.bloom_query(want, other, hashes, k))
+    assert got.mean() <= 1.3 * p
     bl.close()
 
 

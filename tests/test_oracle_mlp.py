@@ -109,9 +109,41 @@ def test_splitmix64_vector():
     assert om.mix64(0x9E3779B97F4A7C15) == 0xE220A8397B1DCDAF
 
 
-def test_bloom_geometry():
-    blocks, hashes = om.bloom_geometry(1_000_000, 0.01)
-    assert hashes == 7
-    assert blocks == int(np.ceil(1.5 * 1_000_000 * np.log(100) / np.log(2) ** 2 / 512))
-    blk, bits = om.bloom_bits(12345, blocks, hashes)
-    assert 0 <= blk < blocks and len(bits) == 7 and all(0 <= b < 512 for b in bits)
+def test_mulhi64_and_mix64_vectorised():
+    rng = np.random.default_rng(3)
+    a = rng.integers(0, 1 << 63, 2000, dtype=np.uint64) * np.uint64(2) + np.uint64(1)
+    b = int(rng.integers(1, 1 << 62)) * 3
+    got = om.mulhi64_np(a, b)
+    want = np.array([(int(x) * b) >> 64 for x in a], dtype=np.uint64)
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(om.mix64_np(a[:50]), np.array([om.mix64(int(x)) for x in a[:50]], np.uint64))
+
+
+@pytest.mark.parametrize("p", [0.01, 0.001, 0.05])
+def test_bloom_sizing_is_minimal_on_its_grid(p):
+    bpk, h = om.bloom_size(p)
+    assert 1 <= h <= 10
+    assert om.bloom_fpr(bpk, h) <= p
+    import math
+    base = -math.log(p) / math.log(2) ** 2
+    j = round(bpk / base / 0.05 - 20)
+    if j > 0:
+        assert om.bloom_fpr(base * (1 + 0.05 * (j - 1)), h) > p
+
+
+def test_bloom_fpr_formula_matches_simulation():
+    """The Poisson model the sizing rests on predicts the measured rate of
+    the filter built by the restated hash scheme."""
+    rng = np.random.default_rng(5)
+    k = 31
+    keys = rng.integers(0, 1 << 62, 200_000, dtype=np.uint64) << np.uint64(2)
+    words, h = om.bloom_geometry(keys.size, 0.01)
+    bits = om.bloom_build(keys, words, h, k)
+    assert om.bloom_query(bits, keys, h, k).all()
+    other = (rng.integers(0, 1 << 62, 1_000_000, dtype=np.uint64) << np.uint64(2)) | np.uint64(1 << 10)
+    measured = om.bloom_query(bits, other, h, k).mean()
+    predicted = om.bloom_fpr(words * 64 / keys.size, h)
+    assert abs(measured - predicted) < 0.15 * predicted, (measured, predicted)
+    assert measured <= 0.0115
+    # low bits below the k-mer are ignored
+    assert om.bloom_query(bits, keys | np.uint64(3), h, k).all()
