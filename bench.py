@@ -62,12 +62,14 @@ def main() -> int:
     ap.add_argument("--no-roof", action="store_true", help="skip the random-gather roof measurement")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 3 times (HBM counter calibration) and exit")
-    ap.add_argument("--workload", choices=["reads", "large", "batched", "prefilter"], default="reads",
+    ap.add_argument("--workload", choices=["reads", "large", "batched", "prefilter", "pangenome"], default="reads",
                     help="reads: the 100 Mbp read search (default); large: a 3 Gbp uniform-random index "
                          "queried with independent 31-mers, 50%% present, through the reference's search "
                          "semantics (the HBM-bound case); batched: 10,000 uneven read sets with 5%% N, "
                          "autotuned, in HIP-graph-captured batches; prefilter: the learned negative "
-                         "prefilter (trained on the GPU) on a 90%%-negative read mix, filtered vs plain")
+                         "prefilter (trained on the GPU) on a 90%%-negative read mix, filtered vs plain; "
+                         "pangenome: 20 haplotypes of a 150 Mbp base (0.1%% SNP, 0.01%% indel), 1B independent "
+                         "31-mers 50%% present, plain and prefilter side by side")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc child runs that measure HBM traffic per launch")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
@@ -82,6 +84,8 @@ def main() -> int:
     ap.add_argument("--threshold", type=float, default=0.0, help="prefilter logit threshold")
     ap.add_argument("--negative-fraction", type=float, default=0.9)
     ap.add_argument("--large-bp", type=int, default=3_000_000_000)
+    ap.add_argument("--base-bp", type=int, default=150_000_000)
+    ap.add_argument("--haplotypes", type=int, default=20)
     ap.add_argument("--queries", type=int, default=1_000_000_000)
     args = ap.parse_args()
 
@@ -95,11 +99,12 @@ def main() -> int:
         # before this process touches the GPU: the counter runs are children
         traffic = measure_traffic(args)
     args.traffic = traffic
-    if args.workload in ("large", "batched", "prefilter"):
+    if args.workload in ("large", "batched", "prefilter", "pangenome"):
         if args.gpus != 1:
             print("bench.py measures one GPU.", file=sys.stderr)
             return 2
-        return {"large": run_large, "batched": run_batched, "prefilter": run_prefilter}[args.workload](args)
+        return {"large": run_large, "batched": run_batched, "prefilter": run_prefilter,
+                "pangenome": run_pangenome}[args.workload](args)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the index is replicated per GPU and queries shard by read set "
@@ -675,6 +680,164 @@ def run_prefilter(args) -> int:
                    "threshold": args.threshold, "threshold_table": thr_table,
                    "bloom_only_bytes": bloom_all.bytes, "bloom_only_zero_fn": bloom_all_zero_fn,
                    "train": pf.stats, "train_total_s": round(train_total_s, 1)},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def run_pangenome(args) -> int:
+    """Pangenome-style large index: `haplotypes` copies of a seeded
+    `base_bp` uniform genome, each with 0.1% substitutions and 0.01% indels
+    (1-10 bp, half insertions), generated and indexed on the GPU.
+    `queries` independent 31-mers: even ones from uniform positions of
+    uniformly chosen haplotypes (present), odd ones uniform random (absent
+    w.h.p.). Timed side by side on the same resident queries: the plain
+    search (the reference's per-k-mer semantics), the learned prefilter
+    (MLP trained on the GPU + Bloom filter over its false negatives) with
+    the masked search, and a Bloom filter over every indexed k-mer with the
+    masked search. Filtered answers must equal the plain ones bit for bit."""
+    import torch
+
+    import sbwt_gpu_amd as sg
+    from sbwt_gpu_amd import _capi, synth
+    from sbwt_gpu_amd.api import _ptr, _stream
+    from sbwt_gpu_amd.learned_filter import random_kmers, train_prefilter
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    k, Q = args.k, args.queries
+    L = _capi.load()
+    s = _stream()
+    t = time.time()
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(31)
+    base = torch.randint(0, 4, (args.base_bp,), dtype=torch.uint8, device=dev, generator=gen)
+    chars, starts = synth.gpu_haplotypes(base, args.haplotypes, seed=32)
+    del base
+    torch.cuda.synchronize()
+    G = chars.numel()
+    log(f"{args.haplotypes} haplotypes, {G} bases generated on the GPU in {time.time() - t:.1f}s")
+    t = time.time()
+    ix = sg.Index.build(chars, k, sg.Config(presearch_depth=args.presearch, layout=args.layout), starts=starts)
+    info = ix.info
+    torch.cuda.synchronize()
+    build_s = time.time() - t
+    log(f"index built on GPU in {build_s:.1f}s: n={info['n']} columns, {info['device_bytes'] / 2**30:.2f} GiB")
+    packed = torch.empty(L.sbwt_packed_words(G), dtype=torch.int64, device=dev)
+    _capi.check(L.sbwt_pack(_ptr(chars), G, _ptr(packed), None, s))
+    del chars
+    starts_h = starts.cpu().numpy()
+    lens = np.diff(starts_h)
+    nh = lens.size
+    half = (Q + 1) // 2
+    hap = torch.randint(0, nh, (half,), device=dev, generator=gen)
+    span = torch.from_numpy(lens - k + 1).to(dev)[hap]
+    off = (torch.rand(half, device=dev, generator=gen, dtype=torch.float64) * span).to(torch.int64)
+    pos = starts[:-1][hap] + off
+    del hap, span, off
+    kmers = torch.empty(Q, dtype=torch.int64, device=dev)
+    kpos = torch.empty(half, dtype=torch.int64, device=dev)
+    _capi.check(L.sbwt_extract_kmers(_ptr(packed), _ptr(pos), half, _ptr(kpos), s))
+    kmers[0::2] = kpos
+    del pos, kpos, packed
+    kmers[1::2] = random_kmers(Q // 2, k, gen)
+    out = torch.empty_like(kmers)
+    maybe = torch.empty(Q, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+    log(f"{Q} queries ready")
+
+    t = time.time()
+    pf = train_prefilter(ix, positives=args.train_positives, negatives=args.train_negatives,
+                         epochs=args.train_epochs, batch=args.train_batch, lr=args.lr, seed=7,
+                         threshold=args.threshold, fpr=0.01, log=log)
+    train_total_s = time.time() - t
+    log(f"prefilter ready in {train_total_s:.1f}s: {pf.stats}")
+    allpos = ix.enumerate_kmers()
+    bloom_all = sg.Bloom(k, allpos.numel(), 0.01)
+    bloom_all.insert(allpos)
+    del allpos
+
+    stream = torch.cuda.current_stream()
+
+    def plain(ev=None):
+        if ev:
+            ev[1].record(stream)
+        _capi.check(L.sbwt_search_kmers(ix._h, _ptr(kmers), Q, _ptr(out), s))
+        if ev:
+            ev[2].record(stream)
+
+    def filtered(kind):
+        def run(ev=None):
+            if ev:
+                ev[0].record(stream)
+            if kind == "learned":
+                pf.maybe(kmers, out=maybe)
+            else:
+                _capi.check(L.sbwt_bloom_query(bloom_all._h, _ptr(kmers), Q, _ptr(maybe), s))
+            if ev:
+                ev[1].record(stream)
+            _capi.check(L.sbwt_search_kmers_masked(ix._h, _ptr(kmers), Q, _ptr(maybe), _ptr(out), 0, s))
+            if ev:
+                ev[2].record(stream)
+        return run
+
+    paths = {"plain": plain, "filtered": filtered("learned"), "bloom_only": filtered("bloom")}
+    results, answers = {}, {}
+    nsteps = max(args.steps, 1)
+    for name, run in paths.items():
+        for _ in range(args.warmup):
+            run()
+        torch.cuda.synchronize()
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(nsteps)]
+        t0 = time.perf_counter()
+        for i in range(nsteps):
+            run(evs[i])
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / nsteps
+        filt_ms = 0.0 if name == "plain" else float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
+        search_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+        answers[name] = out.cpu().numpy().view(np.uint64).copy()
+        mrate = None if name == "plain" else round(float(maybe.float().mean().item()), 4)
+        results[name] = {"kmers_per_s": round(Q / el, 1), "ms_per_step": round(el * 1e3, 3),
+                         "filter_kernel_ms": round(filt_ms, 3), "search_kernel_ms": round(search_ms, 3),
+                         "maybe_fraction": mrate}
+        log(f"{name}: {Q / el / 1e9:.3f} G k-mers/s ({el * 1e3:.2f} ms/step; filter {filt_ms:.2f} ms, "
+            f"search {search_ms:.2f} ms, maybe {mrate})")
+    res = answers["plain"]
+    exact = {n: bool(np.array_equal(a, res)) for n, a in answers.items()}
+    none = np.uint64(0xFFFFFFFFFFFFFFFF)
+    found = int((res != none).sum())
+    even_found = bool((res[0::2] != none).all())
+    log(f"answers equal the plain search: {exact}; found {found}/{Q}; every sampled k-mer found: {even_found}")
+    ix.counters(reset=True)
+    ix.search_kmers(kmers, out=out, count=True)
+    torch.cuda.synchronize()
+    lines = int(ix.counters(reset=True)["lines"])
+    kern_s = results["plain"]["search_kernel_ms"] / 1e3
+    achieved = lines * 128 / kern_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None, "kernel": "k_search_kmers",
+                "kernel_avg_ms": results["plain"]["search_kernel_ms"], "lines_per_launch": lines,
+                "bytes_per_kmer": round(lines * 128 / Q, 2), "index_bytes": info["device_bytes"]}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_kmers(ix, kmers, res, k, args)
+    line = {
+        "metric": METRIC, "value": results["plain"]["kmers_per_s"], "unit": "k-mer queries/s", "n_gpus": 1,
+        "steps": nsteps, "warmup": args.warmup, "ms_per_step": results["plain"]["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32 search, bf16/fp32 MLP",
+        "data": "synthetic (seeded, generated on the GPU)",
+        "config": {"workload": f"pangenome index: {args.haplotypes} haplotypes of a {args.base_bp / 1e6:g} Mbp "
+                               f"uniform base genome (0.1% SNP, 0.01% indel), k={k}, presearch {args.presearch}; "
+                               f"{Q} independent 31-mers, 50% present; plain and prefilter side by side",
+                   "bases": G, "index_columns": info["n"], "index_build_s": round(build_s, 1),
+                   "layout": f"L{info['layout']}", "found_fraction": round(found / Q, 4),
+                   "paths": results, "answers_equal_plain": exact, "sampled_kmers_all_found": even_found,
+                   "filtered_vs_plain": round(results["filtered"]["kmers_per_s"] / results["plain"]["kmers_per_s"], 3),
+                   "bloom_only_vs_plain": round(results["bloom_only"]["kmers_per_s"] /
+                                                results["plain"]["kmers_per_s"], 3),
+                   "bloom_only_bytes": bloom_all.bytes, "train": pf.stats, "train_total_s": round(train_total_s, 1)},
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -152,6 +152,14 @@ int sbwt_index_set_layout(sbwt_index *ix, uint32_t layout, void *stream);
 int sbwt_search_kmers(const sbwt_index *ix, const uint64_t *d_kmers,
                       uint64_t count, uint64_t *d_out, void *stream);
 
+/* sbwt_search_kmers_flags where k-mers with d_maybe[q] == 0 (one byte per
+ * k-mer, e.g. from sbwt_prefilter) answer ~0 without a search. Equals the
+ * unmasked search whenever no indexed k-mer is marked 0. No reference
+ * counterpart (the prefilter's exact stage). */
+int sbwt_search_kmers_masked(const sbwt_index *ix, const uint64_t *d_kmers,
+                             uint64_t count, const uint8_t *d_maybe,
+                             uint64_t *d_out, uint32_t flags, void *stream);
+
 /* The reference kernel ABI (include/gpu/search_kernel.hpp:62-63, launched by
  * launch_search_kernel, include/gpu/search_and_presearch_launchers.hpp:21):
  * d_packed is the 2-bit string of sbwt_pack, d_positions holds count k-mer

@@ -55,6 +55,7 @@ SIGNATURES = {
     "sbwt_kmer_offsets": (i32, [vp, u64, u32, vp, vp]),
     "sbwt_positions": (i32, [vp, vp, u64, u32, vp, vp]),
     "sbwt_extract_kmers": (i32, [vp, vp, u64, vp, vp]),
+    "sbwt_search_kmers_masked": (i32, [vp, vp, u64, vp, vp, u32, vp]),
     "sbwt_search_reads_masked": (i32, [vp, vp, vp, vp, vp, vp, vp, u64, u64, u32, vp, vp, u32, vp]),
     "sbwt_reads_work_bound": (u64, [u64, u64, u32]),
     "sbwt_reads_work": (i32, [vp, u64, u32, vp, vp, vp]),

@@ -173,12 +173,18 @@ class Index:
         check(self.lib.sbwt_index_set_layout(self._h, layout, _stream()))
 
     # ---- search ---------------------------------------------------------------
-    def search_kmers(self, kmers: torch.Tensor, out: torch.Tensor | None = None, count: bool = False) -> torch.Tensor:
-        """kmers: device int64 tensor of top-aligned 2-bit k-mers."""
+    def search_kmers(self, kmers: torch.Tensor, out: torch.Tensor | None = None, count: bool = False,
+                     maybe: torch.Tensor | None = None) -> torch.Tensor:
+        """kmers: device int64 tensor of top-aligned 2-bit k-mers; `maybe`
+        (uint8 per k-mer, e.g. from prefilter()) skips those marked 0."""
         if out is None:
             out = torch.empty_like(kmers)
-        check(self.lib.sbwt_search_kmers_flags(self._h, _ptr(kmers), kmers.numel(), _ptr(out),
-                                               _capi.COUNT if count else 0, _stream()))
+        flags = _capi.COUNT if count else 0
+        if maybe is None:
+            check(self.lib.sbwt_search_kmers_flags(self._h, _ptr(kmers), kmers.numel(), _ptr(out), flags, _stream()))
+        else:
+            check(self.lib.sbwt_search_kmers_masked(self._h, _ptr(kmers), kmers.numel(), _ptr(maybe), _ptr(out),
+                                                    flags, _stream()))
         return out
 
     def counters(self, reset: bool = True) -> dict:

@@ -153,10 +153,11 @@ __device__ __forceinline__ u64 extract1(const u64* __restrict__ w, u64 pos) {
 
 template <int CB, bool COUNT>
 __global__ __launch_bounds__(256) void k_search_kmers(DevIndex ix, const u64* __restrict__ kmers, u64 count,
-                                                      u64* __restrict__ out, unsigned long long* ctr) {
+                                                      const u8* __restrict__ maybe, u64* __restrict__ out,
+                                                      unsigned long long* ctr) {
   const u64 q = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x;
   Tally t;
-  if (q < count) out[q] = full_search<CB, COUNT>(ix, kmers[q], t);
+  if (q < count) out[q] = maybe && !maybe[q] ? kNone : full_search<CB, COUNT>(ix, kmers[q], t);
   flush<COUNT>(ctr, t, q < count ? 1u : 0u);
 }
 
@@ -283,14 +284,14 @@ __global__ __launch_bounds__(256) void k_search_reads(DevIndex ix, const u64* __
 }
 
 template <int CB>
-static int launch_kmers(const sbwt_index* ix, const u64* kmers, u64 count, u64* out, bool count_lines,
-                        hipStream_t s) {
+static int launch_kmers(const sbwt_index* ix, const u64* kmers, u64 count, const u8* maybe, u64* out,
+                        bool count_lines, hipStream_t s) {
   const u32 bt = 256;
   const u64 grid = (count + bt - 1) / bt;
   if (count_lines)
-    k_search_kmers<CB, true><<<grid, bt, 0, s>>>(dev_view(ix), kmers, count, out, ix->d_counters);
+    k_search_kmers<CB, true><<<grid, bt, 0, s>>>(dev_view(ix), kmers, count, maybe, out, ix->d_counters);
   else
-    k_search_kmers<CB, false><<<grid, bt, 0, s>>>(dev_view(ix), kmers, count, out, ix->d_counters);
+    k_search_kmers<CB, false><<<grid, bt, 0, s>>>(dev_view(ix), kmers, count, maybe, out, ix->d_counters);
   SBWT_HIP(hipGetLastError());
   return SBWT_OK;
 }
@@ -347,19 +348,31 @@ using namespace sbwt;
 
 extern "C" {
 
-int sbwt_search_kmers_flags(const sbwt_index* ix, const uint64_t* d_kmers, uint64_t count, uint64_t* d_out,
-                            uint32_t flags, void* stream) {
-  clear_error();
+static int search_kmers_impl(const sbwt_index* ix, const uint64_t* d_kmers, uint64_t count, const uint8_t* d_maybe,
+                             uint64_t* d_out, uint32_t flags, void* stream) {
   SBWT_REQUIRE(ix, "index is NULL");
   if (count == 0) return SBWT_OK;
   SBWT_REQUIRE(d_kmers && d_out, "device buffers must not be NULL");
   const bool cnt = (flags & SBWT_COUNT) != 0;
   hipStream_t s = as_stream(stream);
   switch (ix->layout) {
-    case 16: return launch_kmers<16>(ix, d_kmers, count, d_out, cnt, s);
-    case 32: return launch_kmers<32>(ix, d_kmers, count, d_out, cnt, s);
-    default: return launch_kmers<64>(ix, d_kmers, count, d_out, cnt, s);
+    case 16: return launch_kmers<16>(ix, d_kmers, count, d_maybe, d_out, cnt, s);
+    case 32: return launch_kmers<32>(ix, d_kmers, count, d_maybe, d_out, cnt, s);
+    default: return launch_kmers<64>(ix, d_kmers, count, d_maybe, d_out, cnt, s);
   }
+}
+
+int sbwt_search_kmers_flags(const sbwt_index* ix, const uint64_t* d_kmers, uint64_t count, uint64_t* d_out,
+                            uint32_t flags, void* stream) {
+  clear_error();
+  return search_kmers_impl(ix, d_kmers, count, nullptr, d_out, flags, stream);
+}
+
+int sbwt_search_kmers_masked(const sbwt_index* ix, const uint64_t* d_kmers, uint64_t count, const uint8_t* d_maybe,
+                             uint64_t* d_out, uint32_t flags, void* stream) {
+  clear_error();
+  SBWT_REQUIRE(d_maybe || count == 0, "d_maybe must not be NULL");
+  return search_kmers_impl(ix, d_kmers, count, d_maybe, d_out, flags, stream);
 }
 
 int sbwt_search_kmers(const sbwt_index* ix, const uint64_t* d_kmers, uint64_t count, uint64_t* d_out,

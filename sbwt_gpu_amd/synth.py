@@ -151,3 +151,50 @@ def gpu_read_sets(genome_codes_dev, sets: int, seed: int, max_reads: int = 5000,
         chars[c0:c1] = a
         r0 = r1
     return chars, starts, set_starts
+
+
+def gpu_haplotypes(base_codes_dev, count: int, seed: int, snp: float = 1e-3, indel: float = 1e-4,
+                   max_indel: int = 10):
+    """`count` haplotypes of a base genome (2-bit codes on the device): each
+    base substituted (by one of the three others) with probability `snp`,
+    and indels at rate `indel` per base, half insertions of 1..max_indel
+    random bases and half deletions of 1..max_indel bases, at uniform
+    positions (seeded). Returns (ASCII chars of the haplotypes concatenated,
+    row starts int64[count+1]) on the device, one row per haplotype."""
+    import torch
+
+    dev = base_codes_dev.device
+    G = base_codes_dev.numel()
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    rng = np.random.default_rng(seed)
+    lut = torch.tensor(list(b"ACGT"), dtype=torch.uint8, device=dev)
+    haps, lens = [], []
+    for _ in range(count):
+        flip = torch.rand(G, generator=gen, device=dev) < snp
+        delta = torch.randint(1, 4, (G,), generator=gen, device=dev, dtype=torch.uint8)
+        codes = torch.where(flip, (base_codes_dev + delta) % 4, base_codes_dev)
+        n = int(rng.binomial(G, indel))
+        pos = np.unique(rng.integers(1, G - max_indel, n))
+        L = rng.integers(1, max_indel + 1, pos.size)
+        ins = rng.random(pos.size) < 0.5
+        pool = torch.randint(0, 4, (int(L[ins].sum()) + 1,), generator=gen, device=dev, dtype=torch.uint8)
+        segs, prev, used = [], 0, 0
+        for p, ln, is_ins in zip(pos.tolist(), L.tolist(), ins.tolist()):
+            if p < prev:  # inside the previous deletion
+                continue
+            segs.append(codes[prev:p])
+            if is_ins:
+                segs.append(pool[used:used + ln])
+                used += ln
+                prev = p
+            else:
+                prev = p + ln
+        segs.append(codes[prev:])
+        h = lut[torch.cat(segs).to(torch.int64)]
+        haps.append(h)
+        lens.append(h.numel())
+        del codes, flip, delta
+    starts = torch.zeros(count + 1, dtype=torch.int64, device=dev)
+    starts[1:] = torch.cumsum(torch.tensor(lens, dtype=torch.int64, device=dev), 0)
+    return torch.cat(haps), starts

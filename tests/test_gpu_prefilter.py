@@ -269,3 +269,20 @@ def test_trained_prefilter_zero_false_negatives_and_exact_answers():
     # the filter rejects a useful share of the negatives
     m = maybe.cpu().numpy()
     assert m[plain == NONE].mean() < 0.9
+
+
+def test_masked_kmer_search_equals_plain_where_maybe():
+    rng = np.random.default_rng(21)
+    g = _genome(rng, 50_000)
+    k = 31
+    ix = sg.Index.build([g], k, sg.Config(presearch_depth=8))
+    pos = rng.integers(0, len(g) - k, 3000)
+    q = np.concatenate([rand_kmers(rng, 3000, k),
+                        np.array([int("".join("0123"["ACGT".index(ch)] for ch in g[p:p + k]), 4) << (64 - 2 * k)
+                                  for p in pos], np.uint64)])
+    d = dev_u64(q)
+    plain = sg.to_u64(ix.search_kmers(d))
+    mask = (rng.random(q.size) < 0.5).astype(np.uint8)
+    got = sg.to_u64(ix.search_kmers(d, maybe=torch.from_numpy(mask).cuda()))
+    np.testing.assert_array_equal(got, np.where(mask == 1, plain, NONE))
+    assert (plain[3000:] != NONE).all()
