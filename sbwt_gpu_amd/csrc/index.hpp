@@ -54,8 +54,9 @@ sbwt_index* new_index(const sbwt_config* cfg, u32 k, u64 n);
 // Validates n and k against what the device layout supports.
 int check_shape(u64 n, u32 k);
 // The persistent read search (stream.hip).
-int search_reads_v2(const sbwt_index* ix, const u64* packed, const u64* nmask, const u64* rs, const u64* off,
-                    const u64* work, const u32* items, u64 rows, u64 bound, u32 seg, u64* out, bool nbreak,
-                    bool stream, bool cert, bool count, hipStream_t s);
+int search_reads_persistent(const sbwt_index* ix, const u64* packed, const u64* nmask, const u64* rs,
+                            const u64* off, const u64* work, const u32* items, u64 rows, u64 bound, u32 seg,
+                            const u8* maybe, u64* out, bool nbreak, bool stream, bool cert, bool count,
+                            hipStream_t s);
 
 }  // namespace sbwt

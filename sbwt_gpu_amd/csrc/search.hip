@@ -410,9 +410,10 @@ static int search_reads_impl(const sbwt_index* ix, const uint64_t* d_packed, con
   // streamed or certified (the reference's way)
   const bool stream_mode = !independent && ix->has_groups;
   const bool cert = !independent && !(flags & SBWT_READS_NO_CERT);
-  if ((flags & SBWT_READS_PERSISTENT) && !d_maybe)
-    return search_reads_v2(ix, d_packed, d_nmask, d_row_starts, d_offsets, d_work, d_items, rows, work_bound, seg,
-                           d_out, nbreak, stream_mode, cert, cnt, as_stream(stream));
+  if (flags & SBWT_READS_PERSISTENT)
+    return search_reads_persistent(ix, d_packed, d_nmask, d_row_starts, d_offsets, d_work, d_items, rows,
+                                   work_bound, seg, d_maybe, d_out, nbreak, stream_mode, cert, cnt,
+                                   as_stream(stream));
   const ReadArgs a{d_packed, d_nmask, d_row_starts, d_offsets, d_work, rows, work_bound, seg, d_maybe, d_out};
   hipStream_t s = as_stream(stream);
   switch (ix->layout) {

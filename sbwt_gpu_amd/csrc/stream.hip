@@ -1,11 +1,11 @@
-// Persistent read search of libsbwt_gpu_amd (the default read search).
+// Persistent uniform-step read search of libsbwt_gpu_amd.
 //
 // Same answers as searching every k-mer independently (the reference's
-// search_kernel.hpp:61-104 applied to every position of every row), computed
-// with less work:
+// search_kernel.hpp:61-104 applied to every position of every row), with
+// less work:
 //
-// * Streaming: after k-mer x at node j is found, x[1..k-1]c is answered from
-//   j in one step through the suffix-group start (see search.hip).
+// * Streaming: after k-mer x at node j is found, x[1..k-1]c is the node
+//   C[c] + rank_c(g), g the start of j's suffix group, if g has the c-edge.
 // * Dead-substring certificates: when a search of a string w ends with an
 //   empty interval, no node label ends with w, so no k-mer of the index
 //   contains w (every substring of an indexed k-mer ends some node label:
@@ -16,18 +16,44 @@
 //   after L chars, certifying the k-mers that contain [d, d+L-1]) and the
 //   string ending at d that the remaining k-mers share, so a run of up to k
 //   absent k-mers costs two short searches instead of k.
-// * Lane refill: each lane is a small state machine that issues exactly one
-//   rank-chunk (or presearch) load per loop iteration and fetches a new work
-//   item (a segment of a row) from a global counter as soon as it runs out,
-//   so no lane idles while another finishes a long search.
+//
+// Execution: every lane is a small state machine, and every loop iteration
+// every busy lane performs exactly one step of the same shape: two rank
+// reads in one row, rank_c(a) and rank_c(b). A search step narrows [l, r]
+// with (a, b) = (l, r + 1); a streaming step is the narrowing of [j, j]
+// (the c-edge exists iff rank_c(j + 1) > rank_c(j)); the edge check at a
+// group start g is the narrowing of [g, g]. A presearch lookup and the rare
+// backwards scan for a group start ride on the same load. So the lanes of a
+// wave execute the same instructions whatever they are doing, instead of
+// one lane running a k-step search while the others wait. Lanes take new
+// work items (segments of rows) from their wave's pool, which claims 64
+// items at a time from a global counter.
 #include "index.hpp"
 
 namespace sbwt {
 
 namespace {
 
-enum : u32 { OP_NONE = 0, OP_PRE, OP_STEP, OP_STREAM, OP_GSCAN, OP_GEDGE, OP_DONE };
+enum : u32 { M_IDLE = 0, M_SEARCH, M_STREAM, M_EDGE, M_GSCAN };
 enum : u32 { P_KMER = 0, P_CERTP, P_CERTL };
+constexpr u32 kNoPos = 0xFFFFFFFFu;
+// Work items a wave claims per atomic on the shared counter (at least 64,
+// so one claim serves every lane of a refill). One atomic per lane or per
+// refill serialises on the counter's L2 line and stalls the waves.
+constexpr u64 kItemBatch = 64;
+
+struct ReadsDev {
+  const u64* packed;
+  const u64* nmask;
+  const u64* row_starts;
+  const u64* offsets;
+  const u64* work;
+  const u32* items;
+  const u8* maybe;
+  u64 rows;
+  u32 seg;
+  u32 nbreak, stream, cert;
+};
 
 __device__ __forceinline__ u64 window2(const u64* __restrict__ w, u64 pos) {
   const u64 b = pos << 1;
@@ -39,136 +65,155 @@ __device__ __forceinline__ u32 chunk_line(u32 col) {
   return static_cast<u32>((static_cast<u64>(col / Layout<CB>::COLS) * CB) >> 7);
 }
 
-struct ReadsDev {
-  const u64* packed;
-  const u64* nmask;
-  const u64* row_starts;
-  const u64* offsets;
-  const u64* work;
-  const u32* items;
-  u64 rows;
-  u32 seg;
-};
-
 }  // namespace
 
 __global__ void k_zero_counter(unsigned long long* c) { *c = 0ull; }
 
-template <int CB, bool NBREAK, bool STREAM, bool CERT, bool COUNT>
-__global__ __launch_bounds__(256) void k_reads_v2(DevIndex ix, ReadsDev rd, unsigned long long* __restrict__ items,
-                                                  u64* __restrict__ out, unsigned long long* ctr) {
+template <int CB, bool COUNT>
+__global__ __launch_bounds__(256) void k_reads_uniform(DevIndex ix, ReadsDev rd,
+                                                       unsigned long long* __restrict__ item_ctr,
+                                                       u64* __restrict__ out, unsigned long long* ctr) {
   constexpr u32 COLS = Layout<CB>::COLS;
   const u32 k = ix.k, m = ix.m;
+  const u32 lane = threadIdx.x & 63;
   const u64 total_items = rd.rows ? rd.work[rd.rows] : 0;
 
-  u32 op = OP_NONE;
-  // work item
+  // work item: k-mers [j, jend) of a row of rowk k-mers starting at character cbase
   u64 cbase = 0, obase = 0;
-  u32 j = 0, jend = 0, rowk = 0;
+  u32 j = 0, jend = 0, rowk = 0, rowlen = 0;
+  bool live = true;
+  // current step
+  u32 mode = M_IDLE, purpose = P_KMER;
+  u32 l = 0, r = 0, i = 0, len = 0, s = 0;  // search state (M_GSCAN: l = chunk, r = last offset)
+  u64 win = 0;                             // top-aligned window of the searched string
   bool prev = false;
-  u32 pnode = 0;
-  u32 clo = 1, chi = 0;  // certified k-mers [clo, chi] of the row
-  u32 kpos = ~0u, knode = 0;
-  // rolling code window of k-mer j: packed words a, a+1
-  // (kNoWord + 1 is no word index either, so a fresh lane always loads both)
-  constexpr u64 kNoWord = ~0ull - 1;
-  u64 wa = 0, wb = 0, widx = kNoWord;
-  // search state
-  u64 win = 0;
-  u32 s = 0, len = 0, i = 0, l = 0, r = 0, purpose = 0;
-  // certificate / stream state
-  u32 cp = 0, sc = 0, gch = 0, glim = 0, gpos = 0;
+  u32 pnode = 0, sc = 0;
+  // certified-absent k-mers [clo, chi]; a k-mer at kpos already found at knode
+  u32 clo = 1, chi = 0, kpos = kNoPos, knode = 0, cp = 0;
+  u32 nN = kNoPos;  // first non-ACGT position >= the last scan point (row-relative), kNoPos = unknown
+  u32 scanned = 0;  // positions < scanned hold no non-ACGT character past nN
+  // packed word holding the last streamed character
+  u64 cw = 0, cw_idx = ~0ull;
   u32 t_lines = 0, t_steps = 0, t_kmers = 0, t_search = 0;
 
-  auto code_window = [&](u32 jj) -> u64 {
-    const u64 b = (cbase + jj) << 1;
-    const u64 a = b >> 6;
-    if (a != widx) {
-      if (a == widx + 1) {
-        wa = wb;
-      } else {
-        wa = rd.packed[a];
-      }
-      wb = rd.packed[a + 1];
-      widx = a;
-    }
-    return (wa << (b & 63)) | ((wb >> (63 - (b & 63))) >> 1);
-  };
-  auto n_window = [&](u32 jj) -> u64 {
-    const u64 b = cbase + jj;
-    return (rd.nmask[b >> 6] << (b & 63)) | ((rd.nmask[(b >> 6) + 1] >> (63 - (b & 63))) >> 1);
-  };
-  auto start_search = [&](u32 pur, u32 start, u32 length, u64 w) {
+  auto start_search = [&](u32 pur, u32 start, u32 length) {
     purpose = pur;
     s = start;
     len = length;
-    win = w;
+    win = window2(rd.packed, cbase + start);
+    i = 0;
+    mode = M_SEARCH;
     if (COUNT) t_search++;
-    if (m && length >= m) {
-      op = OP_PRE;
-    } else {
+    if (!(m && length >= m)) {
       l = 0;
       r = ix.C[4] - 1u;
-      i = 0;
-      op = OP_STEP;
     }
-  };
-  // a k-mer before j was found absent with a dead string ending at char d
-  auto start_cert = [&](u32 d) {
-    if (!CERT) return;
-    if (clo <= chi && chi >= j) return;  // a pending range still covers what follows
-    if (d < j || d >= rowk) return;      // nothing ahead to certify, or no k-mer starts at d
-    cp = d;
-    start_search(P_CERTP, d, k, window2(rd.packed, cbase + d));
-  };
-  auto fetch = [&]() -> bool {
-    const u64 it = atomicAdd(items, 1ull);
-    if (it >= total_items) return false;
-    u64 lo = 0;
-    if (rd.items) {
-      lo = rd.items[it];
-    } else {
-      u64 hi = rd.rows;
-      while (hi - lo > 1) {
-        const u64 mid = (lo + hi) >> 1;
-        if (rd.work[mid] <= it) lo = mid; else hi = mid;
-      }
-    }
-    const u64 o0 = rd.offsets[lo];
-    rowk = static_cast<u32>(rd.offsets[lo + 1] - o0);
-    j = static_cast<u32>(it - rd.work[lo]) * rd.seg;
-    jend = rowk - j < rd.seg ? rowk : j + rd.seg;
-    cbase = rd.row_starts[lo];
-    obase = o0;
-    prev = false;
-    clo = 1;
-    chi = 0;
-    kpos = ~0u;
-    widx = kNoWord;
-    return true;
   };
   auto answer = [&](u64 v) {
     out[obase + j] = v;
     j++;
     if (COUNT) t_kmers++;
   };
-
-  while (true) {
-    // ---- advance to this lane's next memory operation ----
-    while (op == OP_NONE) {
-      if (j >= jend) {
-        if (!fetch()) {
-          op = OP_DONE;
+  // a k-mer before j was found absent with a dead string ending at char d
+  auto start_cert = [&](u32 d) {
+    if (!rd.cert) return;
+    if (clo <= chi && chi >= j) return;  // a pending range still covers what follows
+    if (d < j || d >= rowk) return;      // nothing ahead to certify, or no k-mer starts at d
+    cp = d;
+    start_search(P_CERTP, d, k);
+  };
+  // does the k-mer at j contain a non-ACGT character?
+  auto has_n = [&]() -> bool {
+    if (nN == kNoPos || nN < j) {
+      if (scanned <= j) scanned = j;
+      // next non-ACGT at or after `scanned`, within the row
+      u64 p = cbase + scanned;
+      const u64 end = cbase + rowlen;
+      nN = rowlen;
+      while (p < end) {
+        const u64 x = rd.nmask[p >> 6] & (~0ull >> (p & 63));
+        if (x) {
+          const u64 q = (p & ~63ull) + static_cast<u64>(__clzll(x));
+          if (q < end) nN = static_cast<u32>(q - cbase);
           break;
         }
-        continue;
+        p = (p | 63) + 1;
       }
-      if (j >= clo && j <= chi) {
+      scanned = nN;
+    }
+    return nN <= j + k - 1;
+  };
+
+  u64 pool_next = 0, pool_end = 0;  // the wave's unclaimed items (same in every lane)
+  for (;;) {
+    // ---- refill: lanes without work take the next items
+    const bool want = live && mode == M_IDLE && j >= jend;
+    const u64 wm = __ballot(want);
+    if (wm) {
+      // take items from the wave's pool; refill it kItemBatch items at a time
+      const u64 nw = static_cast<u64>(__popcll(wm));
+      const u64 rank = static_cast<u64>(__popcll(wm & ((1ull << lane) - 1)));
+      const u64 avail = pool_end - pool_next;
+      u64 it;
+      if (avail >= nw) {
+        it = pool_next + rank;
+        pool_next += nw;
+      } else {
+        const u32 leader = static_cast<u32>(__ffsll(static_cast<unsigned long long>(wm)) - 1);
+        u64 nb = 0;
+        if (lane == leader) nb = atomicAdd(item_ctr, static_cast<unsigned long long>(kItemBatch));
+        nb = __shfl(nb, static_cast<int>(leader));
+        it = rank < avail ? pool_next + rank : nb + (rank - avail);
+        pool_next = nb + (nw - avail);
+        pool_end = nb + kItemBatch;
+      }
+      if (want) {
+        if (it >= total_items) {
+          live = false;
+        } else {
+          u64 row = 0;
+          if (rd.items) {
+            row = rd.items[it];
+          } else {
+            u64 hi = rd.rows;
+            while (hi - row > 1) {
+              const u64 mid = (row + hi) >> 1;
+              if (rd.work[mid] <= it) row = mid; else hi = mid;
+            }
+          }
+          const u64 o0 = rd.offsets[row];
+          rowk = static_cast<u32>(rd.offsets[row + 1] - o0);
+          j = static_cast<u32>(it - rd.work[row]) * rd.seg;
+          jend = rowk - j < rd.seg ? rowk : j + rd.seg;
+          cbase = rd.row_starts[row];
+          rowlen = rowk + k - 1;
+          obase = o0;
+          prev = false;
+          clo = 1;
+          chi = 0;
+          kpos = kNoPos;
+          nN = kNoPos;
+          scanned = 0;
+        }
+      }
+    }
+    // ---- advance: answer what needs no index access, then set up one step
+    while (mode == M_IDLE && j < jend) {
+      if (j >= clo && j <= chi) {  // a certified run: all of it at once
+        const u32 e = chi + 1 < jend ? chi + 1 : jend;
+        if (COUNT) t_kmers += e - j;
+        for (; j < e; j++) out[obase + j] = kNone;
         prev = false;
-        answer(kNone);
         continue;
       }
-      if (NBREAK && (n_window(j) >> (64 - k)) != 0) {
+      if (rd.nbreak && has_n()) {  // the k-mers j..nN all contain the character at nN
+        const u32 e = nN + 1 < jend ? nN + 1 : jend;
+        if (COUNT) t_kmers += e - j;
+        for (; j < e; j++) out[obase + j] = kNone;
+        prev = false;
+        continue;
+      }
+      if (rd.maybe && !rd.maybe[obase + j]) {
         prev = false;
         answer(kNone);
         continue;
@@ -179,170 +224,129 @@ __global__ __launch_bounds__(256) void k_reads_v2(DevIndex ix, ReadsDev rd, unsi
         answer(knode);
         continue;
       }
-      const u64 w = code_window(j);
-      if (STREAM && prev) {
-        sc = static_cast<u32>(w >> (64 - 2 * k)) & 3u;
-        op = OP_STREAM;
+      if (rd.stream && prev) {
+        const u64 p = cbase + j + k - 1;  // the new character
+        if ((p >> 5) != cw_idx) {
+          cw_idx = p >> 5;
+          cw = rd.packed[cw_idx];
+        }
+        sc = static_cast<u32>(cw >> (62 - 2 * (p & 31))) & 3u;
+        mode = M_STREAM;
       } else {
-        start_search(P_KMER, j, k, w);
+        start_search(P_KMER, j, k);
       }
     }
-    if (!__any(op != OP_DONE)) break;
+    if (!__any(mode != M_IDLE || live)) break;
 
-    // ---- memory phase: one load (or two chunks of one step) per lane ----
-    const uint4* pa = nullptr;
-    const uint4* pb = nullptr;
-    const uint2* pp = nullptr;
-    u32 c = 0, ca = 0, cb = 0;
-    if (op == OP_PRE) {
-      pp = ix.pre + (win >> (64 - 2 * m));
-    } else if (op == OP_STEP) {
-      c = static_cast<u32>(win >> (62 - 2 * i)) & 3u;
-      ca = l / COLS;
-      cb = (r + 1u) / COLS;
-      pa = row_of(ix, c) + static_cast<u64>(ca) * Layout<CB>::Q;
-      pb = row_of(ix, c) + static_cast<u64>(cb) * Layout<CB>::Q;
-    } else if (op == OP_STREAM) {
-      c = sc;
-      ca = pnode / COLS;
-      pa = row_of(ix, c) + static_cast<u64>(ca) * Layout<CB>::Q;
-    } else if (op == OP_GSCAN) {
-      pa = row_of(ix, kGroupRow) + static_cast<u64>(gch) * Layout<CB>::Q;
-    } else if (op == OP_GEDGE) {
-      c = sc;
-      ca = gpos / COLS;
-      pa = row_of(ix, c) + static_cast<u64>(ca) * Layout<CB>::Q;
-    }
+    // ---- one step, executed by every lane in straight-line code (idle lanes
+    // read chunk 0 of row 0 and keep their state): two rank reads in one
+    // row; a presearch lookup reads the 16-byte block holding its entry.
+    const bool busy = mode != M_IDLE, srch = mode == M_SEARCH, gscan = mode == M_GSCAN;
+    const bool edge = mode == M_EDGE, strm = mode == M_STREAM;
+    const bool pre = srch && i == 0 && m && len >= m;
+    const u32 c = srch ? static_cast<u32>(win >> (62 - 2 * i)) & 3u : sc;
+    const u32 a = !busy ? 0u : (srch ? l : (gscan ? l * COLS : (edge ? l : pnode)));
+    const u32 b = srch ? r + 1u : (gscan || !busy ? a : a + 1u);
+    const uint4* rowp = row_of(ix, gscan ? kGroupRow : (busy ? c : 0u));
+    const u32 ca = a / COLS, cb = b / COLS;
+    const u64 pi = pre ? win >> (64 - 2 * m) : 0;
     Chunk<CB> qa, qb;
-    uint2 e = make_uint2(0, 0);
-    if (pa) {
+    {
+      const uint4* pa = pre ? reinterpret_cast<const uint4*>(ix.pre) + (pi >> 1)
+                            : rowp + static_cast<u64>(ca) * Layout<CB>::Q;
+      const uint4* pb = rowp + static_cast<u64>(cb) * Layout<CB>::Q;
 #pragma unroll
       for (u32 t = 0; t < Layout<CB>::Q; t++) {
-        const uint4 v = pa[t];
-        qa.w[4 * t] = v.x;
-        qa.w[4 * t + 1] = v.y;
-        qa.w[4 * t + 2] = v.z;
-        qa.w[4 * t + 3] = v.w;
+        const uint4 va = pa[pre ? 0 : t];
+        const uint4 vb = pb[t];
+        qa.w[4 * t] = va.x;
+        qa.w[4 * t + 1] = va.y;
+        qa.w[4 * t + 2] = va.z;
+        qa.w[4 * t + 3] = va.w;
+        qb.w[4 * t] = vb.x;
+        qb.w[4 * t + 1] = vb.y;
+        qb.w[4 * t + 2] = vb.z;
+        qb.w[4 * t + 3] = vb.w;
       }
     }
-    if (pb) {
-#pragma unroll
-      for (u32 t = 0; t < Layout<CB>::Q; t++) {
-        const uint4 v = pb[t];
-        qb.w[4 * t] = v.x;
-        qb.w[4 * t + 1] = v.y;
-        qb.w[4 * t + 2] = v.z;
-        qb.w[4 * t + 3] = v.w;
-      }
+    const u32 oa = a - ca * COLS, ob = b - cb * COLS;
+    const u32 ra = rank_chunk<CB>(qa, oa), rb = rank_chunk<CB>(qb, ob);
+    const u32 base = c_of(ix, c);
+    if (COUNT) {
+      if (pre) t_lines++;
+      else if (srch) t_lines += chunk_line<CB>(a) == chunk_line<CB>(b) ? 1u : 2u;
+      else if (edge) t_lines += chunk_line<CB>(a) != chunk_line<CB>(pnode) ? 1u : 0u;
+      else if (busy) t_lines++;
+      if (strm || (srch && !pre)) t_steps++;
     }
-    if (pp) e = *pp;
-
-    // ---- compute phase ----
-    bool sdone = false, salive = false;  // a search finished; alive = survived len chars
-    u32 sdead = 0;                        // dead after this many chars
-    if (op == OP_PRE) {
-      if (COUNT) t_lines++;
-      l = e.x;
-      r = e.y;
-      i = m;
-      if (l > r) {
-        sdone = true;
-        sdead = m;
-      } else if (i >= len) {
-        sdone = salive = true;
-      } else {
-        op = OP_STEP;
-      }
-    } else if (op == OP_STEP) {
-      if (COUNT) {
-        t_lines += chunk_line<CB>(l) == chunk_line<CB>(r + 1u) ? 1u : 2u;
-        t_steps++;
-      }
-      const u32 base = c_of(ix, c);
-      l = base + rank_chunk<CB>(qa, l - ca * COLS);
-      r = base + rank_chunk<CB>(qb, (r + 1u) - cb * COLS) - 1u;
-      i++;
-      if (l > r) {
-        sdone = true;
-        sdead = i;
-      } else if (i >= len) {
-        sdone = salive = true;
-      }
-    } else if (op == OP_STREAM) {
-      if (COUNT) {
-        t_lines++;
-        t_steps++;
-      }
-      const u32 o = pnode - ca * COLS;
-      if (bit_chunk<CB>(qa, o)) {
-        pnode = c_of(ix, c) + rank_chunk<CB>(qa, o);
-        answer(pnode);
-        op = OP_NONE;
-      } else {
-        gch = ca;
-        glim = o;
-        op = OP_GSCAN;
-      }
-    } else if (op == OP_GSCAN) {
-      if (COUNT) t_lines++;
-      const int p = last_one_upto<CB>(qa, glim);
-      if (p >= 0 || gch == 0) {
-        const u32 g = p >= 0 ? gch * COLS + static_cast<u32>(p) : 0u;
-        if (g == pnode) {
-          prev = false;
-          answer(kNone);
-          op = OP_NONE;
-          start_cert(j - 1 + k - 1);
-        } else {
-          gpos = g;
-          op = OP_GEDGE;
-        }
-      } else {
-        gch--;
-        glim = COLS - 1;
-      }
-    } else if (op == OP_GEDGE) {
-      if (COUNT && chunk_line<CB>(gpos) != chunk_line<CB>(pnode)) t_lines++;
-      const u32 go = gpos - ca * COLS;
-      op = OP_NONE;
-      if (bit_chunk<CB>(qa, go)) {
-        pnode = c_of(ix, c) + rank_chunk<CB>(qa, go);
-        answer(pnode);
-      } else {
-        prev = false;
-        answer(kNone);
-        start_cert(j - 1 + k - 1);
-      }
+    // search: the narrowed interval
+    const u32 sl = pre ? ((pi & 1) ? qa.w[2] : qa.w[0]) : base + ra;
+    const u32 sr = pre ? ((pi & 1) ? qa.w[3] : qa.w[1]) : base + rb - 1u;
+    const u32 si = pre ? m : i + 1;
+    const bool sdead = srch && sl > sr, salive = srch && sl <= sr && si >= len;
+    // stream / edge: the narrowing of [a, a]
+    const bool found = (strm || edge) && rb > ra;
+    const u32 node = srch ? sl : base + ra;
+    // common outcomes, by selects: a k-mer answered now, a search continuing
+    const bool kdone = (srch && purpose == P_KMER && (sdead || salive)) || found || (edge && !found);
+    const bool kfound = salive || found;
+    if (srch) {
+      l = sl;
+      r = sr;
+      i = si;
     }
-
-    if (sdone) {
-      op = OP_NONE;
-      if (purpose == P_KMER) {
-        if (salive) {
-          prev = true;
-          pnode = l;
-          answer(l);
-        } else {
-          prev = false;
-          answer(kNone);
-          start_cert(s + sdead - 1);
-        }
-      } else if (purpose == P_CERTP) {
-        if (salive) {
-          kpos = cp;  // the k-mer at cp is present; answered when j reaches it
-          knode = l;
-        } else {
-          const u32 lo = cp + sdead > k ? cp + sdead - k : 0u;
-          clo = lo > j ? lo : j;
-          chi = cp;
-          if (lo > j && cp - lo + 2 <= k) {
-            // the k-mers j..lo-1 all contain [lo-1, cp]
-            const u32 q = lo - 1;
-            start_search(P_CERTL, q, cp - q + 1, window2(rd.packed, cbase + q));
+    if (kdone) {
+      out[obase + j] = kfound ? static_cast<u64>(node) : kNone;
+      j++;
+      if (COUNT) t_kmers++;
+      prev = kfound;
+      pnode = node;
+    }
+    const bool settled = kdone || (srch && (sdead || salive)) || (strm && !found);
+    mode = settled ? M_IDLE : mode;
+    // rarer outcomes: certificates, group scans
+    const bool kfail = kdone && !kfound;
+    const bool cdone = srch && purpose != P_KMER && (sdead || salive);
+    if (kfail || cdone || gscan || (strm && !found)) {
+      if (kfail) {
+        start_cert(srch ? s + i - 1 : j - 1 + k - 1);
+      } else if (cdone) {
+        if (purpose == P_CERTP) {
+          if (salive) {
+            kpos = cp;  // the k-mer at cp is present; answered when j reaches it
+            knode = l;
+          } else {
+            const u32 lo = cp + i > k ? cp + i - k : 0u;
+            clo = lo > j ? lo : j;
+            chi = cp;
+            if (lo > j && cp - lo + 2 <= k) start_search(P_CERTL, lo - 1, cp - lo + 2);  // [lo-1, cp]
           }
+        } else if (sdead) {  // P_CERTL: the k-mers j..lo-1 all contain [lo-1, cp]
+          clo = j;
         }
-      } else {  // P_CERTL
-        if (!salive) clo = j;
+      } else if (strm) {
+        // no c-edge at j: find j's group start g (column 0 starts a group)
+        mode = M_GSCAN;
+        l = ca;
+        r = oa;
+      } else {  // M_GSCAN
+        if (COUNT) t_lines++;
+        const int p = last_one_upto<CB>(qa, r);
+        if (p >= 0 || l == 0) {
+          const u32 g = p >= 0 ? l * COLS + static_cast<u32>(p) : 0u;
+          if (g == pnode) {  // j starts its group and has no c-edge
+            mode = M_IDLE;
+            prev = false;
+            answer(kNone);
+            start_cert(j - 1 + k - 1);
+          } else {
+            mode = M_EDGE;
+            l = g;
+          }
+        } else {
+          l--;
+          r = COLS - 1;
+        }
       }
     }
   }
@@ -353,54 +357,38 @@ __global__ __launch_bounds__(256) void k_reads_v2(DevIndex ix, ReadsDev rd, unsi
     for (int q = 0; q < 4; q++) {
       u64 x = v[q];
       for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
-      if ((threadIdx.x & 63) == 0) atomicAdd(ctr + q, static_cast<unsigned long long>(x));
+      if (lane == 0) atomicAdd(ctr + q, static_cast<unsigned long long>(x));
     }
   }
 }
 
-template <int CB, bool NBREAK, bool STREAM, bool CERT, bool COUNT>
-static int launch_v2_5(const sbwt_index* ix, const ReadsDev& rd, u64 bound, u64* out, hipStream_t s) {
+template <int CB, bool COUNT>
+static int launch_uniform(const sbwt_index* ix, const ReadsDev& rd, u64 bound, u64* out, hipStream_t s) {
   const u32 bt = 256;
+  int per = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_reads_uniform<CB, COUNT>, bt, 0) != hipSuccess ||
+      per < 1)
+    per = 4;
   u64 grid = (bound + bt - 1) / bt;
-  const u64 cap = static_cast<u64>(ix->num_cus ? ix->num_cus : 256) * 8;
+  const u64 cap = static_cast<u64>(ix->num_cus ? ix->num_cus : 256) * static_cast<u64>(per);
   if (grid > cap) grid = cap;
   if (grid == 0) grid = 1;
   k_zero_counter<<<1, 1, 0, s>>>(ix->d_item_ctr);
-  k_reads_v2<CB, NBREAK, STREAM, CERT, COUNT><<<grid, bt, 0, s>>>(dev_view(ix), rd, ix->d_item_ctr, out,
-                                                                 ix->d_counters);
+  k_reads_uniform<CB, COUNT><<<grid, bt, 0, s>>>(dev_view(ix), rd, ix->d_item_ctr, out, ix->d_counters);
   SBWT_HIP(hipGetLastError());
   return SBWT_OK;
 }
 
-template <int CB, bool NBREAK, bool STREAM, bool CERT>
-static int launch_v2_4(const sbwt_index* ix, const ReadsDev& rd, u64 bound, u64* out, bool count, hipStream_t s) {
-  return count ? launch_v2_5<CB, NBREAK, STREAM, CERT, true>(ix, rd, bound, out, s)
-               : launch_v2_5<CB, NBREAK, STREAM, CERT, false>(ix, rd, bound, out, s);
-}
-
-template <int CB>
-static int launch_v2(const sbwt_index* ix, const ReadsDev& rd, u64 bound, u64* out, bool nbreak, bool stream,
-                     bool cert, bool count, hipStream_t s) {
-  if (nbreak) {
-    if (stream) return cert ? launch_v2_4<CB, true, true, true>(ix, rd, bound, out, count, s)
-                            : launch_v2_4<CB, true, true, false>(ix, rd, bound, out, count, s);
-    return cert ? launch_v2_4<CB, true, false, true>(ix, rd, bound, out, count, s)
-                : launch_v2_4<CB, true, false, false>(ix, rd, bound, out, count, s);
-  }
-  if (stream) return cert ? launch_v2_4<CB, false, true, true>(ix, rd, bound, out, count, s)
-                          : launch_v2_4<CB, false, true, false>(ix, rd, bound, out, count, s);
-  return cert ? launch_v2_4<CB, false, false, true>(ix, rd, bound, out, count, s)
-              : launch_v2_4<CB, false, false, false>(ix, rd, bound, out, count, s);
-}
-
-int search_reads_v2(const sbwt_index* ix, const u64* packed, const u64* nmask, const u64* rs, const u64* off,
-                    const u64* work, const u32* items, u64 rows, u64 bound, u32 seg, u64* out, bool nbreak,
-                    bool stream, bool cert, bool count, hipStream_t s) {
-  const ReadsDev rd{packed, nmask, rs, off, work, items, rows, seg};
+int search_reads_persistent(const sbwt_index* ix, const u64* packed, const u64* nmask, const u64* rs,
+                            const u64* off, const u64* work, const u32* items, u64 rows, u64 bound, u32 seg,
+                            const u8* maybe, u64* out, bool nbreak, bool stream, bool cert, bool count,
+                            hipStream_t s) {
+  const ReadsDev rd{packed, nmask, rs, off, work, items, maybe, rows, seg,
+                    nbreak ? 1u : 0u, stream ? 1u : 0u, cert ? 1u : 0u};
   switch (ix->layout) {
-    case 16: return launch_v2<16>(ix, rd, bound, out, nbreak, stream, cert, count, s);
-    case 32: return launch_v2<32>(ix, rd, bound, out, nbreak, stream, cert, count, s);
-    default: return launch_v2<64>(ix, rd, bound, out, nbreak, stream, cert, count, s);
+    case 16: return count ? launch_uniform<16, true>(ix, rd, bound, out, s) : launch_uniform<16, false>(ix, rd, bound, out, s);
+    case 32: return count ? launch_uniform<32, true>(ix, rd, bound, out, s) : launch_uniform<32, false>(ix, rd, bound, out, s);
+    default: return count ? launch_uniform<64, true>(ix, rd, bound, out, s) : launch_uniform<64, false>(ix, rd, bound, out, s);
   }
 }
 
