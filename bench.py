@@ -52,11 +52,13 @@ def main() -> int:
     ap.add_argument("--presearch", type=int, default=12)
     ap.add_argument("--layout", type=int, default=16)
     ap.add_argument("--seg", type=int, default=256)
-    ap.add_argument("--mode", choices=["default", "nocert", "persistent", "persistent-nocert", "independent"],
+    ap.add_argument("--mode", choices=["default", "nocert", "persistent", "persistent-nocert", "independent",
+                                       "persistent-independent", "lanes"],
                     default="default",
-                    help="read search variant: one lane per read segment with streaming + certificates "
-                         "(default), without certificates, the persistent lane-refill kernel (with or without "
-                         "certificates), or every k-mer searched independently as the reference does")
+                    help="read search variant: streaming + certificates on the uniform-step kernel (default), "
+                         "without certificates (per-lane kernel), the uniform-step kernel forced (with or "
+                         "without certificates, or independent), every k-mer searched independently as the "
+                         "reference does, or the per-lane kernel with certificates (lanes)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-roof", action="store_true", help="skip the random-gather roof measurement")
@@ -156,7 +158,9 @@ def main() -> int:
     items = torch.empty(bound, dtype=torch.int32, device=dev)
     flags = {"default": 0, "nocert": _capi.READS_NO_CERT, "persistent": _capi.READS_PERSISTENT,
              "persistent-nocert": _capi.READS_PERSISTENT | _capi.READS_NO_CERT,
-             "independent": _capi.READS_INDEPENDENT}[args.mode]
+             "independent": _capi.READS_INDEPENDENT,
+             "persistent-independent": _capi.READS_PERSISTENT | _capi.READS_INDEPENDENT,
+             "lanes": _capi.READS_SEGMENT_LANES}[args.mode]
     check = _capi.check
     s = _stream()
     stream = torch.cuda.current_stream()
@@ -206,7 +210,7 @@ def main() -> int:
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic_bytes(args.traffic),
                 "traffic_detail": args.traffic,
-                "kernel": "k_reads_v2" if args.mode.startswith("persistent") else "k_search_reads", "kernel_avg_ms": round(kern_avg_s * 1e3, 4),
+                "kernel": "k_reads_uniform" if args.mode in ("default", "persistent", "persistent-nocert", "persistent-independent") else "k_search_reads", "kernel_avg_ms": round(kern_avg_s * 1e3, 4),
                 "lines_per_launch": lines, "bytes_per_kmer": round(alg_bytes / total, 2),
                 "index_bytes": info["device_bytes"]}
     if not args.no_roof:
@@ -914,7 +918,7 @@ def measure_traffic(args):
     if not prof:
         return {"error": "rocprofv3 not found"}
     kern = "k_search_kmers" if args.workload == "large" else (
-        "k_reads_v2" if args.mode.startswith("persistent") else "k_search_reads")
+        "k_reads_uniform" if args.mode in ("default", "persistent", "persistent-nocert", "persistent-independent") else "k_search_reads")
     child = [sys.executable, "-u", os.path.abspath(__file__), "--pmc-child", "--workload", args.workload,
              "--mode", args.mode, "--layout", str(args.layout), "--presearch", str(args.presearch),
              "--seg", str(args.seg), "--genome-bp", str(args.genome_bp), "--reads", str(args.reads),

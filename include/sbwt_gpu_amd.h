@@ -48,10 +48,14 @@ extern "C" {
                                      consecutive k-mers through suffix groups
                                      (when the index has them) and skip k-mers
                                      that contain a substring proven absent */
-#define SBWT_READS_PERSISTENT 8u  /* persistent lane-refill kernel (each lane a
-                                     state machine issuing one rank load per
-                                     iteration); default: one lane per work
-                                     item, k-mers in order */
+#define SBWT_READS_PERSISTENT 8u  /* the persistent uniform-step kernel (each
+                                     lane a state machine taking one
+                                     two-rank step per iteration, work items
+                                     claimed per wave); the default when
+                                     streaming with certificates */
+#define SBWT_READS_SEGMENT_LANES 32u /* one lane per work item, k-mers in
+                                     order; the default for independent
+                                     searches or without certificates */
 #define SBWT_READS_NO_CERT 16u    /* no dead-substring certificates (for
                                      measurement) */
 #define SBWT_COUNT 4u             /* counted variant: add the 128-byte lines the
@@ -191,9 +195,8 @@ int sbwt_positions(const uint64_t *d_row_starts, const uint64_t *d_offsets,
 
 /* sbwt_search_reads where k-mers with d_maybe[offset] == 0 (one byte per
  * k-mer in output order, e.g. from sbwt_prefilter) answer ~0 without a
- * search. Uses the one-lane-per-segment kernel (SBWT_READS_PERSISTENT is
- * ignored). The answers equal the unmasked search whenever no indexed
- * k-mer is marked 0. */
+ * search. The answers equal the unmasked search whenever no indexed k-mer
+ * is marked 0. No reference counterpart (the prefilter's exact stage). */
 int sbwt_search_reads_masked(const sbwt_index *ix, const uint64_t *d_packed,
                              const uint64_t *d_nmask, const uint64_t *d_row_starts,
                              const uint64_t *d_offsets, const uint64_t *d_work,

@@ -16,6 +16,7 @@ READS_INDEPENDENT = 2
 COUNT = 4
 READS_PERSISTENT = 8
 READS_NO_CERT = 16
+READS_SEGMENT_LANES = 32
 
 # every function the header declares, with (restype, argtypes)
 vp, u64, u32, i32 = C.c_void_p, C.c_uint64, C.c_uint32, C.c_int

@@ -221,7 +221,8 @@ class Index:
 
     def search_reads(self, reads: "ReadBatch", out: torch.Tensor | None = None, n_as_a: bool = False,
                      independent: bool = False, count: bool = False, persistent: bool = False,
-                     certificates: bool = True, maybe: torch.Tensor | None = None) -> torch.Tensor:
+                     certificates: bool = True, maybe: torch.Tensor | None = None,
+                     lanes: bool = False) -> torch.Tensor:
         """Answers for every k-mer of the batch; `maybe` (uint8, one per k-mer,
         e.g. from prefilter()) skips the k-mers it marks 0."""
         if reads.k != self.k:
@@ -230,7 +231,7 @@ class Index:
             out = torch.empty(max(reads.total_kmers, 1), dtype=torch.int64, device=reads.chars.device)
         flags = ((_capi.READS_N_AS_A if n_as_a else 0) | (_capi.READS_INDEPENDENT if independent else 0)
                  | (_capi.COUNT if count else 0) | (_capi.READS_PERSISTENT if persistent else 0)
-                 | (0 if certificates else _capi.READS_NO_CERT))
+                 | (0 if certificates else _capi.READS_NO_CERT) | (_capi.READS_SEGMENT_LANES if lanes else 0))
         args = (self._h, _ptr(reads.packed), _ptr(reads.nmask), _ptr(reads.row_starts), _ptr(reads.offsets),
                 _ptr(reads.work), _ptr(reads.items), reads.rows, reads.work_bound, reads.seg)
         if maybe is None:
@@ -347,10 +348,10 @@ def format_results(results: np.ndarray, offsets: np.ndarray) -> bytes:
     return buf.raw[: size.value]
 
 
-def _flags(n_as_a=False, independent=False, count=False, persistent=False, certificates=True) -> int:
+def _flags(n_as_a=False, independent=False, count=False, persistent=False, certificates=True, lanes=False) -> int:
     return ((_capi.READS_N_AS_A if n_as_a else 0) | (_capi.READS_INDEPENDENT if independent else 0)
             | (_capi.COUNT if count else 0) | (_capi.READS_PERSISTENT if persistent else 0)
-            | (0 if certificates else _capi.READS_NO_CERT))
+            | (0 if certificates else _capi.READS_NO_CERT) | (_capi.READS_SEGMENT_LANES if lanes else 0))
 
 
 class ReadsPlan:

@@ -2,7 +2,7 @@
 
 Times the read search of a sample batch under every combination of rank
 layout (16/32/64-byte chunks), presearch depth, segment length (k-mers per
-work item) and kernel (one lane per segment, or the persistent lane-refill
+work item) and kernel (one lane per segment, or the persistent uniform-step
 kernel), checks that every combination gives the same answers, and leaves
 the index in the fastest layout and depth. The reference fixes these by
 flags (superblock size, presearch depth and threads per block,
@@ -43,7 +43,7 @@ def autotune(index: Index, chars: torch.Tensor, row_starts: torch.Tensor, *, lay
             index.presearch(m)
         for seg, kern in itertools.product(segs, kernels):
             plan = ReadsPlan(rows, n_chars, seg)
-            flags = {"persistent": kern == "persistent", "n_as_a": n_as_a}
+            flags = {"persistent": kern == "persistent", "lanes": kern == "lane", "n_as_a": n_as_a}
             plan.search(index, chars, row_starts, out, **flags)  # warm
             times = []
             for _ in range(reps):

@@ -410,7 +410,12 @@ static int search_reads_impl(const sbwt_index* ix, const uint64_t* d_packed, con
   // streamed or certified (the reference's way)
   const bool stream_mode = !independent && ix->has_groups;
   const bool cert = !independent && !(flags & SBWT_READS_NO_CERT);
-  if (flags & SBWT_READS_PERSISTENT)
+  // the uniform-step kernel wins where certificates mix short searches into
+  // the streams; the per-lane kernel's tight loop wins for independent
+  // searches and streams without certificates
+  const bool uniform = (flags & SBWT_READS_PERSISTENT) ||
+                       (!(flags & SBWT_READS_SEGMENT_LANES) && stream_mode && cert);
+  if (uniform)
     return search_reads_persistent(ix, d_packed, d_nmask, d_row_starts, d_offsets, d_work, d_items, rows,
                                    work_bound, seg, d_maybe, d_out, nbreak, stream_mode, cert, cnt,
                                    as_stream(stream));

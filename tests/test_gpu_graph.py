@@ -93,4 +93,5 @@ def test_autotune_keeps_answers_and_applies_best():
     assert ix.info["layout"] == best["layout"] and ix.info["presearch_depth"] == best["presearch_depth"]
     batch = sg.ReadBatch(chars, starts, 31, seg=best["segment_kmers"])
     want = O.search_reads(oix, chars.cpu().numpy(), starts.cpu().numpy().astype(np.uint64), 31)
-    np.testing.assert_array_equal(sg.to_u64(ix.search_reads(batch, persistent=best["kernel"] == "persistent")), want)
+    np.testing.assert_array_equal(sg.to_u64(ix.search_reads(batch, persistent=best["kernel"] == "persistent",
+                                                            lanes=best["kernel"] == "lane")), want)

@@ -177,7 +177,8 @@ def _reads(rng, g, n_reads, err=0.01, nrate=0.0, lens=(150, 150)):
 
 
 READ_VARIANTS = [{}, {"certificates": False}, {"persistent": True}, {"persistent": True, "certificates": False},
-                 {"independent": True}, {"independent": True, "persistent": True}]
+                 {"independent": True}, {"independent": True, "persistent": True}, {"lanes": True},
+                 {"lanes": True, "certificates": False}]
 
 
 @pytest.mark.parametrize("layout", [16, 32, 64])

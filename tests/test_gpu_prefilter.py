@@ -226,7 +226,8 @@ def _reads(rng, g, count, L, err=0.01):
     return out
 
 
-@pytest.mark.parametrize("variant", [{}, {"certificates": False}, {"independent": True}])
+@pytest.mark.parametrize("variant", [{}, {"certificates": False}, {"independent": True}, {"lanes": True},
+                                     {"persistent": True, "independent": True}])
 def test_masked_search_equals_plain_where_maybe(variant):
     rng = np.random.default_rng(9)
     g = _genome(rng, 100_000)
