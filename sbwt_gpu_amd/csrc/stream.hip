@@ -53,6 +53,7 @@ struct ReadsDev {
   u64 rows;
   u32 seg;
   u32 nbreak, stream, cert;
+  u32 vec_out;  // out is 16-byte aligned: full 32-byte answer groups go out as two 16-byte stores
 };
 
 __device__ __forceinline__ u64 window2(const u64* __restrict__ w, u64 pos) {
@@ -92,6 +93,10 @@ struct Walker {
   u32 clo = 1, chi = 0, kpos = kNoPos, knode = 0, cp = 0;
   u32 nN = kNoPos;  // first non-ACGT position (row-relative) at or after j, if >= j; kNoPos = unknown
   u64 cw = 0, cw_idx = ~0ull;  // packed word holding the last streamed character
+  // answers of the aligned group of four output slots bg (32 bytes), held in
+  // LDS (obuf[s * 256 + threadIdx.x], s = slot), slots present in bmask
+  u64 bg = ~0ull;
+  u32 bmask = 0;
 
   // One step's reads: chunks at columns a and b of one row, or a presearch
   // entry. Only the addresses live across the loads; update() recomputes the
@@ -156,10 +161,43 @@ struct Walker {
     }
   }
 
-  __device__ __forceinline__ void answer(u64* __restrict__ out, UTally& t, u64 v) {
-    out[obase + j] = v;
+  // Answers are written per aligned group of four slots: a full group as two
+  // 16-byte stores (a whole 32-byte sector), a partial one slot by slot. A
+  // lane writing 8 bytes per k-mer into its own rows otherwise leaves partial
+  // lines that the L2 evicts and writes back piecemeal.
+  __device__ __forceinline__ void flush(u64* __restrict__ out, const ReadsDev& rd, u64* obuf) {
+    if (!bmask) return;
+    u64* o = out + bg * 4;
+    const u64 b0 = obuf[0], b1 = obuf[256], b2 = obuf[512], b3 = obuf[768];
+    if (bmask == 15 && rd.vec_out) {
+      reinterpret_cast<uint4*>(o)[0] = make_uint4(static_cast<u32>(b0), static_cast<u32>(b0 >> 32),
+                                                  static_cast<u32>(b1), static_cast<u32>(b1 >> 32));
+      reinterpret_cast<uint4*>(o)[1] = make_uint4(static_cast<u32>(b2), static_cast<u32>(b2 >> 32),
+                                                  static_cast<u32>(b3), static_cast<u32>(b3 >> 32));
+    } else {
+      if (bmask & 1) o[0] = b0;
+      if (bmask & 2) o[1] = b1;
+      if (bmask & 4) o[2] = b2;
+      if (bmask & 8) o[3] = b3;
+    }
+    bmask = 0;
+  }
+
+  // the answer for the k-mer at j; moves to j + 1
+  __device__ __forceinline__ void answer(u64* __restrict__ out, const ReadsDev& rd, u64* obuf, UTally& t,
+                                         u64 v) {
+    const u64 q = obase + j;
+    const u64 g = q >> 2;
+    const u32 sl = static_cast<u32>(q) & 3u;
+    if (g != bg) {
+      flush(out, rd, obuf);
+      bg = g;
+    }
+    obuf[sl * 256] = v;
+    bmask |= 1u << sl;
     j++;
     if (COUNT) t.kmers++;
+    if (bmask == 15 || j >= jend) flush(out, rd, obuf);
   }
 
   // a k-mer before j was found absent with a dead string ending at char d
@@ -194,45 +232,38 @@ struct Walker {
 
   // Answer what needs no index access, then set up one step.
   __device__ __forceinline__ void advance(const DevIndex& ix, const ReadsDev& rd, u64* __restrict__ out,
-                                          UTally& t) {
+                                          u64* obuf, UTally& t) {
     const u32 k = ix.k;
     while (mode == M_IDLE && j < jend) {
-      if (j >= clo && j <= chi) {  // a certified run: all of it at once
-        const u32 e = chi + 1 < jend ? chi + 1 : jend;
-        if (COUNT) t.kmers += e - j;
-        for (; j < e; j++) out[obase + j] = kNone;
+      // one k-mer per pass: answered here (certified, non-ACGT, filtered out,
+      // or already found by a certificate search), or handed to a step
+      bool emit = true;
+      u64 v = kNone;
+      if (j >= clo && j <= chi) {
         prev = false;
-        continue;
-      }
-      if (rd.nbreak && has_n(rd, k)) {  // the k-mers j..nN all contain the character at nN
-        const u32 e = nN + 1 < jend ? nN + 1 : jend;
-        if (COUNT) t.kmers += e - j;
-        for (; j < e; j++) out[obase + j] = kNone;
+      } else if (rd.nbreak && has_n(rd, k)) {
         prev = false;
-        continue;
-      }
-      if (rd.maybe && !rd.maybe[obase + j]) {
+      } else if (rd.maybe && !rd.maybe[obase + j]) {
         prev = false;
-        answer(out, t, kNone);
-        continue;
-      }
-      if (j == kpos) {
+      } else if (j == kpos) {
         prev = true;
         pnode = knode;
-        answer(out, t, knode);
-        continue;
-      }
-      if (rd.stream && prev) {
-        const u64 p = cbase + j + k - 1;  // the new character
-        if ((p >> 5) != cw_idx) {
-          cw_idx = p >> 5;
-          cw = rd.packed[cw_idx];
-        }
-        sc = static_cast<u32>(cw >> (62 - 2 * (p & 31))) & 3u;
-        mode = M_STREAM;
+        v = knode;
       } else {
-        start_search(ix, rd, t, P_KMER, j, k);
+        emit = false;
+        if (rd.stream && prev) {
+          const u64 p = cbase + j + k - 1;  // the new character
+          if ((p >> 5) != cw_idx) {
+            cw_idx = p >> 5;
+            cw = rd.packed[cw_idx];
+          }
+          sc = static_cast<u32>(cw >> (62 - 2 * (p & 31))) & 3u;
+          mode = M_STREAM;
+        } else {
+          start_search(ix, rd, t, P_KMER, j, k);
+        }
       }
+      if (emit) answer(out, rd, obuf, t, v);
     }
   }
 
@@ -270,7 +301,7 @@ struct Walker {
 
   // Apply the step: the common outcomes by selects, the rare ones in one branch.
   __device__ __forceinline__ void update(const DevIndex& ix, const ReadsDev& rd, u64* __restrict__ out,
-                                         UTally& t, const Chunk<CB>& qa, const Chunk<CB>& qb) {
+                                         u64* obuf, UTally& t, const Chunk<CB>& qa, const Chunk<CB>& qb) {
     const u32 k = ix.k;
     const Cols st = cols(ix);
     const u64 pi = st.pre ? win >> (64 - 2 * ix.m) : 0;
@@ -301,10 +332,9 @@ struct Walker {
       r = sr;
       i = si;
     }
+    bool emit = kdone;
+    u64 ev = kfound ? static_cast<u64>(node) : kNone;
     if (kdone) {
-      out[obase + j] = kfound ? static_cast<u64>(node) : kNone;
-      j++;
-      if (COUNT) t.kmers++;
       prev = kfound;
       pnode = node;
     }
@@ -313,6 +343,7 @@ struct Walker {
     // rarer outcomes: certificates, group scans
     const bool kfail = kdone && !kfound;
     const bool cdone = srch && purpose != P_KMER && (sdead || salive);
+    if (emit) answer(out, rd, obuf, t, ev);
     if (kfail || cdone || gscan || (strm && !found)) {
       if (kfail) {
         // a k-mer search of the k-mer at j died after i characters; a
@@ -345,7 +376,7 @@ struct Walker {
           if (g == pnode) {  // j starts its group and has no c-edge
             mode = M_IDLE;
             prev = false;
-            answer(out, t, kNone);
+            answer(out, rd, obuf, t, kNone);  // (rare: a second call site)
             start_cert(ix, rd, t, j - 1 + k - 1);
           } else {
             mode = M_EDGE;
@@ -402,6 +433,9 @@ __global__ __launch_bounds__(256) void k_reads_uniform(DevIndex ix, ReadsDev rd,
                                                        unsigned long long* __restrict__ item_ctr,
                                                        u64* __restrict__ out, unsigned long long* ctr) {
   using W = Walker<CB, COUNT>;
+  static_assert(P == 1, "one LDS answer buffer per lane");
+  __shared__ u64 obuf_all[4 * 256];
+  u64* obuf = obuf_all + threadIdx.x;
   const u32 lane = threadIdx.x & 63;
   const u64 total_items = rd.rows ? rd.work[rd.rows] : 0;
   W w[P];
@@ -412,7 +446,7 @@ __global__ __launch_bounds__(256) void k_reads_uniform(DevIndex ix, ReadsDev rd,
 #pragma unroll
     for (int p = 0; p < P; p++) {
       refill(w[p], pool, item_ctr, total_items, rd, lane);
-      w[p].advance(ix, rd, out, t);
+      w[p].advance(ix, rd, out, obuf, t);
       active = active || w[p].mode != M_IDLE || w[p].live;
     }
     if (!__any(active)) break;
@@ -423,7 +457,7 @@ __global__ __launch_bounds__(256) void k_reads_uniform(DevIndex ix, ReadsDev rd,
 #pragma unroll
     for (int p = 0; p < P; p++) W::fetch(st[p], qa[p], qb[p]);
 #pragma unroll
-    for (int p = 0; p < P; p++) w[p].update(ix, rd, out, t, qa[p], qb[p]);
+    for (int p = 0; p < P; p++) w[p].update(ix, rd, out, obuf, t, qa[p], qb[p]);
   }
   if (COUNT) {
     u64 v[4] = {t.lines, t.steps, t.kmers, t.searches};
@@ -463,7 +497,8 @@ int search_reads_persistent(const sbwt_index* ix, const u64* packed, const u64* 
                             const u8* maybe, u64* out, bool nbreak, bool stream, bool cert, bool count,
                             hipStream_t s) {
   const ReadsDev rd{packed, nmask, rs, off, work, items, maybe, rows, seg,
-                    nbreak ? 1u : 0u, stream ? 1u : 0u, cert ? 1u : 0u};
+                    nbreak ? 1u : 0u, stream ? 1u : 0u, cert ? 1u : 0u,
+                    (reinterpret_cast<uintptr_t>(out) & 15) == 0 ? 1u : 0u};
   switch (ix->layout) {
     case 16: return count ? launch_uniform<16, true>(ix, rd, bound, out, s) : launch_uniform<16, false>(ix, rd, bound, out, s);
     case 32: return count ? launch_uniform<32, true>(ix, rd, bound, out, s) : launch_uniform<32, false>(ix, rd, bound, out, s);
