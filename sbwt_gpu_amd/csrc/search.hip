@@ -9,6 +9,7 @@
 // start g of j (the first node sharing x's last k-1 characters) carries the
 // only c-edge into it: the answer is C[c] + rank_c(g) if g has a c-edge,
 // else absent. This yields the same node ids as the full search.
+#include "answers.hpp"
 #include "index.hpp"
 
 namespace sbwt {
@@ -209,7 +210,9 @@ __global__ __launch_bounds__(256) void k_search_reads(DevIndex ix, const u64* __
                                                       const u64* __restrict__ offsets,
                                                       const u64* __restrict__ work, u64 rows, u64 bound, u32 seg,
                                                       const u8* __restrict__ maybe, u64* __restrict__ out,
-                                                      unsigned long long* ctr) {
+                                                      u32 vec_out, unsigned long long* ctr) {
+  __shared__ u64 obuf[4 * kAnswerStride];
+  AnswerBuffer ab{obuf + threadIdx.x};
   const u64 gid = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x;
   Tally t;
   u32 done = 0;
@@ -225,7 +228,7 @@ __global__ __launch_bounds__(256) void k_search_reads(DevIndex ix, const u64* __
     const u64 j0 = (gid - work[r]) * seg;
     const u64 cnt = (rowk - j0) < seg ? (rowk - j0) : seg;
     const u64 rs = row_starts[r];
-    u64* o = out + offsets[r];
+    const u64 obase = offsets[r];
     const u8* mb = maybe ? maybe + offsets[r] : nullptr;
     const u32 k = ix.k;
     bool prev = false;
@@ -257,7 +260,7 @@ __global__ __launch_bounds__(256) void k_search_reads(DevIndex ix, const u64* __
       }
       prev = res != kNone;
       pnode = static_cast<u32>(res);
-      o[j] = res;
+      ab.put(out, vec_out != 0, obase + j, res, j + 1 == j0 + cnt);
       if (CERT && d > j && d < rowk && !(clo <= chi && chi > j)) {
         u32 lp;
         const u64 rp = search_len<CB, COUNT>(ix, extract2(packed, rs + d), k, t, lp);
@@ -314,9 +317,10 @@ struct ReadArgs {
 
 template <int CB, bool STREAM, bool NBREAK, bool CERT, bool COUNT>
 static int launch_reads5(const sbwt_index* ix, const ReadArgs& a, hipStream_t s) {
-  const u32 bt = 256;
+  const u32 bt = kAnswerStride;  // the answer buffer's LDS layout assumes this block size
   k_search_reads<CB, STREAM, NBREAK, CERT, COUNT><<<(a.bound + bt - 1) / bt, bt, 0, s>>>(
-      dev_view(ix), a.packed, a.nmask, a.rs, a.off, a.work, a.rows, a.bound, a.seg, a.maybe, a.out, ix->d_counters);
+      dev_view(ix), a.packed, a.nmask, a.rs, a.off, a.work, a.rows, a.bound, a.seg, a.maybe, a.out,
+      (reinterpret_cast<uintptr_t>(a.out) & 15) == 0 ? 1u : 0u, ix->d_counters);
   SBWT_HIP(hipGetLastError());
   return SBWT_OK;
 }

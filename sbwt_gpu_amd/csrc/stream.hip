@@ -28,6 +28,7 @@
 // one lane running a k-step search while the others wait. Lanes take new
 // work items (segments of rows) from their wave's pool, which claims 64
 // items at a time from a global counter.
+#include "answers.hpp"
 #include "index.hpp"
 
 namespace sbwt {
@@ -53,7 +54,7 @@ struct ReadsDev {
   u64 rows;
   u32 seg;
   u32 nbreak, stream, cert;
-  u32 vec_out;  // out is 16-byte aligned: full 32-byte answer groups go out as two 16-byte stores
+  u32 vec_out;  // out is 16-byte aligned (full answer groups go out as two 16-byte stores)
 };
 
 __device__ __forceinline__ u64 window2(const u64* __restrict__ w, u64 pos) {
@@ -93,11 +94,6 @@ struct Walker {
   u32 clo = 1, chi = 0, kpos = kNoPos, knode = 0, cp = 0;
   u32 nN = kNoPos;  // first non-ACGT position (row-relative) at or after j, if >= j; kNoPos = unknown
   u64 cw = 0, cw_idx = ~0ull;  // packed word holding the last streamed character
-  // answers of the aligned group of four output slots bg (32 bytes), held in
-  // LDS (obuf[s * 256 + threadIdx.x], s = slot), slots present in bmask
-  u64 bg = ~0ull;
-  u32 bmask = 0;
-
   // One step's reads: chunks at columns a and b of one row, or a presearch
   // entry. Only the addresses live across the loads; update() recomputes the
   // rest from the walker's state.
@@ -161,43 +157,12 @@ struct Walker {
     }
   }
 
-  // Answers are written per aligned group of four slots: a full group as two
-  // 16-byte stores (a whole 32-byte sector), a partial one slot by slot. A
-  // lane writing 8 bytes per k-mer into its own rows otherwise leaves partial
-  // lines that the L2 evicts and writes back piecemeal.
-  __device__ __forceinline__ void flush(u64* __restrict__ out, const ReadsDev& rd, u64* obuf) {
-    if (!bmask) return;
-    u64* o = out + bg * 4;
-    const u64 b0 = obuf[0], b1 = obuf[256], b2 = obuf[512], b3 = obuf[768];
-    if (bmask == 15 && rd.vec_out) {
-      reinterpret_cast<uint4*>(o)[0] = make_uint4(static_cast<u32>(b0), static_cast<u32>(b0 >> 32),
-                                                  static_cast<u32>(b1), static_cast<u32>(b1 >> 32));
-      reinterpret_cast<uint4*>(o)[1] = make_uint4(static_cast<u32>(b2), static_cast<u32>(b2 >> 32),
-                                                  static_cast<u32>(b3), static_cast<u32>(b3 >> 32));
-    } else {
-      if (bmask & 1) o[0] = b0;
-      if (bmask & 2) o[1] = b1;
-      if (bmask & 4) o[2] = b2;
-      if (bmask & 8) o[3] = b3;
-    }
-    bmask = 0;
-  }
-
-  // the answer for the k-mer at j; moves to j + 1
-  __device__ __forceinline__ void answer(u64* __restrict__ out, const ReadsDev& rd, u64* obuf, UTally& t,
+  // the answer for the k-mer at j (through the lane's answer buffer); moves to j + 1
+  __device__ __forceinline__ void answer(u64* __restrict__ out, const ReadsDev& rd, AnswerBuffer& ab, UTally& t,
                                          u64 v) {
-    const u64 q = obase + j;
-    const u64 g = q >> 2;
-    const u32 sl = static_cast<u32>(q) & 3u;
-    if (g != bg) {
-      flush(out, rd, obuf);
-      bg = g;
-    }
-    obuf[sl * 256] = v;
-    bmask |= 1u << sl;
+    ab.put(out, rd.vec_out != 0, obase + j, v, j + 1 >= jend);
     j++;
     if (COUNT) t.kmers++;
-    if (bmask == 15 || j >= jend) flush(out, rd, obuf);
   }
 
   // a k-mer before j was found absent with a dead string ending at char d
@@ -232,7 +197,7 @@ struct Walker {
 
   // Answer what needs no index access, then set up one step.
   __device__ __forceinline__ void advance(const DevIndex& ix, const ReadsDev& rd, u64* __restrict__ out,
-                                          u64* obuf, UTally& t) {
+                                          AnswerBuffer& ab, UTally& t) {
     const u32 k = ix.k;
     while (mode == M_IDLE && j < jend) {
       // one k-mer per pass: answered here (certified, non-ACGT, filtered out,
@@ -263,7 +228,7 @@ struct Walker {
           start_search(ix, rd, t, P_KMER, j, k);
         }
       }
-      if (emit) answer(out, rd, obuf, t, v);
+      if (emit) answer(out, rd, ab, t, v);
     }
   }
 
@@ -301,7 +266,7 @@ struct Walker {
 
   // Apply the step: the common outcomes by selects, the rare ones in one branch.
   __device__ __forceinline__ void update(const DevIndex& ix, const ReadsDev& rd, u64* __restrict__ out,
-                                         u64* obuf, UTally& t, const Chunk<CB>& qa, const Chunk<CB>& qb) {
+                                         AnswerBuffer& ab, UTally& t, const Chunk<CB>& qa, const Chunk<CB>& qb) {
     const u32 k = ix.k;
     const Cols st = cols(ix);
     const u64 pi = st.pre ? win >> (64 - 2 * ix.m) : 0;
@@ -343,7 +308,7 @@ struct Walker {
     // rarer outcomes: certificates, group scans
     const bool kfail = kdone && !kfound;
     const bool cdone = srch && purpose != P_KMER && (sdead || salive);
-    if (emit) answer(out, rd, obuf, t, ev);
+    if (emit) answer(out, rd, ab, t, ev);
     if (kfail || cdone || gscan || (strm && !found)) {
       if (kfail) {
         // a k-mer search of the k-mer at j died after i characters; a
@@ -376,7 +341,7 @@ struct Walker {
           if (g == pnode) {  // j starts its group and has no c-edge
             mode = M_IDLE;
             prev = false;
-            answer(out, rd, obuf, t, kNone);  // (rare: a second call site)
+            answer(out, rd, ab, t, kNone);  // (rare: a second call site)
             start_cert(ix, rd, t, j - 1 + k - 1);
           } else {
             mode = M_EDGE;
@@ -434,8 +399,8 @@ __global__ __launch_bounds__(256) void k_reads_uniform(DevIndex ix, ReadsDev rd,
                                                        u64* __restrict__ out, unsigned long long* ctr) {
   using W = Walker<CB, COUNT>;
   static_assert(P == 1, "one LDS answer buffer per lane");
-  __shared__ u64 obuf_all[4 * 256];
-  u64* obuf = obuf_all + threadIdx.x;
+  __shared__ u64 obuf[4 * kAnswerStride];
+  AnswerBuffer ab{obuf + threadIdx.x};
   const u32 lane = threadIdx.x & 63;
   const u64 total_items = rd.rows ? rd.work[rd.rows] : 0;
   W w[P];
@@ -446,7 +411,7 @@ __global__ __launch_bounds__(256) void k_reads_uniform(DevIndex ix, ReadsDev rd,
 #pragma unroll
     for (int p = 0; p < P; p++) {
       refill(w[p], pool, item_ctr, total_items, rd, lane);
-      w[p].advance(ix, rd, out, obuf, t);
+      w[p].advance(ix, rd, out, ab, t);
       active = active || w[p].mode != M_IDLE || w[p].live;
     }
     if (!__any(active)) break;
@@ -457,7 +422,7 @@ __global__ __launch_bounds__(256) void k_reads_uniform(DevIndex ix, ReadsDev rd,
 #pragma unroll
     for (int p = 0; p < P; p++) W::fetch(st[p], qa[p], qb[p]);
 #pragma unroll
-    for (int p = 0; p < P; p++) w[p].update(ix, rd, out, obuf, t, qa[p], qb[p]);
+    for (int p = 0; p < P; p++) w[p].update(ix, rd, out, ab, t, qa[p], qb[p]);
   }
   if (COUNT) {
     u64 v[4] = {t.lines, t.steps, t.kmers, t.searches};
@@ -476,7 +441,7 @@ constexpr int kWalkersPerLane = 1;
 
 template <int CB, bool COUNT>
 static int launch_uniform(const sbwt_index* ix, const ReadsDev& rd, u64 bound, u64* out, hipStream_t s) {
-  const u32 bt = 256;
+  const u32 bt = kAnswerStride;  // the answer buffer's LDS layout assumes this block size
   constexpr int P = kWalkersPerLane;
   int per = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_reads_uniform<CB, COUNT, P>, bt, 0) != hipSuccess ||

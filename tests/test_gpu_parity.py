@@ -202,6 +202,26 @@ def test_search_reads_matches_oracle(layout, seg):
             np.testing.assert_array_equal(got, want, err_msg=f"n_as_a={n_as_a} {kw}")
 
 
+def test_search_reads_unaligned_output():
+    """Answers written through an output pointer that is 8- but not
+    16-byte aligned (the per-slot store path) equal the aligned ones."""
+    rng = np.random.default_rng(31)
+    g = genome(rng, 60_000)
+    k = 31
+    s = O.SBWT([g], k)
+    reads = _reads(rng, g, 300, err=0.02, nrate=0.01, lens=(31, 200))
+    chars, rs = O.parse_lines("\n".join(reads).encode())
+    oix = oracle_index(s, 9)
+    want = O.search_reads(oix, chars, rs, k)
+    ix = sg.Index.from_rows(s.rows, s.n, k, group=s.group, config=sg.Config(presearch_depth=9))
+    batch = sg.ReadBatch.from_host(reads, k, seg=17)
+    for kw in READ_VARIANTS:
+        buf = torch.full((batch.total_kmers + 3,), 7, dtype=torch.int64, device="cuda")
+        got = ix.search_reads(batch, out=buf[1:batch.total_kmers + 1], **kw)
+        np.testing.assert_array_equal(sg.to_u64(got), want, err_msg=str(kw))
+        assert int(buf[0]) == 7 and int(buf[-1]) == 7 and int(buf[-2]) == 7
+
+
 def test_index_without_group_row():
     """Without suffix-group starts every k-mer is searched (certificates still
     apply)."""
