@@ -164,16 +164,18 @@ int build_layout(sbwt_index* ix, hipStream_t s) {
 // keeps l = r + 1 (include/gpu/presearch_kernel.hpp:19-37).
 template <int CB>
 __global__ void k_presearch(DevIndex ix, u32 m, u64 count, uint2* __restrict__ out) {
-  const u64 idx = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x;
-  if (idx >= count) return;
-  u32 c = static_cast<u32>(idx >> (2 * (m - 1))) & 3u;
-  u32 l = c_of(ix, c);
-  u32 r = (c == 3 ? ix.C[4] : c_of(ix, c + 1)) - 1u;
-  for (int sh = 2 * static_cast<int>(m) - 4; sh >= 0; sh -= 2) {
-    c = static_cast<u32>(idx >> sh) & 3u;
-    narrow<CB>(ix, c, l, r);
+  // grid-stride: a dispatch holds fewer than 2^32 work-items, and m = 16 has 2^32 entries
+  const u64 stride = gridDim.x * static_cast<u64>(blockDim.x);
+  for (u64 idx = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x; idx < count; idx += stride) {
+    u32 c = static_cast<u32>(idx >> (2 * (m - 1))) & 3u;
+    u32 l = c_of(ix, c);
+    u32 r = (c == 3 ? ix.C[4] : c_of(ix, c + 1)) - 1u;
+    for (int sh = 2 * static_cast<int>(m) - 4; sh >= 0; sh -= 2) {
+      c = static_cast<u32>(idx >> sh) & 3u;
+      narrow<CB>(ix, c, l, r);
+    }
+    out[idx] = make_uint2(l, r);
   }
-  out[idx] = make_uint2(l, r);
 }
 
 static int run_presearch(sbwt_index* ix, u32 m, hipStream_t s) {
@@ -187,7 +189,8 @@ static int run_presearch(sbwt_index* ix, u32 m, hipStream_t s) {
   SBWT_HIP(hipMalloc(&ix->d_pre, count * sizeof(uint2)));
   DevIndex v = dev_view(ix);
   const u32 bt = 256;
-  const u64 grid = (count + bt - 1) / bt;
+  const u64 want = (count + bt - 1) / bt;
+  const u64 grid = want < (1ull << 22) ? want : (1ull << 22);
   switch (ix->layout) {
     case 16: k_presearch<16><<<grid, bt, 0, s>>>(v, m, count, ix->d_pre); break;
     case 32: k_presearch<32><<<grid, bt, 0, s>>>(v, m, count, ix->d_pre); break;

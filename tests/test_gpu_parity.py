@@ -104,6 +104,25 @@ def test_search_kmers_one_mbp_scale(layout, m):
     assert (got != NONE).sum() >= 20_000
 
 
+def test_deepest_presearch_table():
+    """The largest presearch depth (16: a 32-bit table index, 4^16 entries)
+    answers like a shallow table; depth 17 is refused."""
+    rng = np.random.default_rng(16)
+    g = genome(rng, 50_000)
+    k = 20
+    s = O.SBWT([g], k)
+    q = topk(sample_queries(rng, g, k, 4000))
+    ix = sg.Index.from_rows(s.rows, s.n, k, group=s.group, config=sg.Config(presearch_depth=6))
+    want = sg.to_u64(ix.search_kmers(dev_u64(q)))
+    np.testing.assert_array_equal(want, oracle_index(s, 6).search(q, k))
+    ix.presearch(16)
+    assert ix.info["presearch_depth"] == 16
+    np.testing.assert_array_equal(sg.to_u64(ix.search_kmers(dev_u64(q))), want)
+    with pytest.raises(sg.SbwtError, match="presearch depth"):
+        ix.presearch(17)
+    ix.close()
+
+
 def test_random_rows_stress_rank():
     """Rows of random bits (total ones n-1, so C[4] = n): not an SBWT, but the
     narrowing is still well defined, and random k-mers drive rank through
