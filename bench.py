@@ -218,6 +218,11 @@ def main() -> int:
                 "kernel": "k_reads_uniform" if args.mode in ("default", "persistent", "persistent-nocert", "persistent-independent") else "k_search_reads", "kernel_avg_ms": round(kern_avg_s * 1e3, 4),
                 "lines_per_launch": lines, "bytes_per_kmer": round(alg_bytes / total, 2),
                 "index_bytes": info["device_bytes"]}
+    fetch = ((args.traffic or {}).get("counters") or {}).get("FETCH_SIZE")
+    if fetch:
+        roofline["hbm_fetch_bytes_per_kmer"] = round(fetch["bytes"] / total, 2)
+    if (args.traffic or {}).get("tcc"):
+        roofline["tcc_hit_rate"] = args.traffic["tcc"]["hit_rate"]
     if not args.no_roof:
         roof = gather_roof()
         roofline["measured_random_roof"] = roof
@@ -328,6 +333,11 @@ def run_large(args) -> int:
                 "traffic_detail": args.traffic, "kernel": "k_search_kmers",
                 "kernel_avg_ms": round(kern_avg_s * 1e3, 3), "lines_per_launch": lines,
                 "bytes_per_kmer": round(lines * 128 / Q, 2), "index_bytes": info["device_bytes"]}
+    fetch = ((args.traffic or {}).get("counters") or {}).get("FETCH_SIZE")
+    if fetch:
+        roofline["hbm_fetch_bytes_per_kmer"] = round(fetch["bytes"] / args.traffic["queries_per_launch"], 2)
+    if (args.traffic or {}).get("tcc"):
+        roofline["tcc_hit_rate"] = args.traffic["tcc"]["hit_rate"]
     if not args.no_roof:
         roof = gather_roof()
         roofline["measured_random_roof"] = roof
@@ -910,11 +920,12 @@ def cpu_baseline_kmers(ix, kmers, res_gpu, k, args):
 
 
 def measure_traffic(args):
-    """HBM bytes per launch of the search kernel from two rocprofv3 --pmc
-    child runs (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) of this
-    workload with --pmc-child (which also copies 1 GiB three times); each
-    counter is scaled by the factor that makes it read 1 GiB for that copy
-    (HIP's device-to-device copy kernel, __amd_rocclr_copyBuffer)."""
+    """HBM bytes per launch of the search kernel from rocprofv3 --pmc child
+    runs of this workload with --pmc-child (which also copies 1 GiB three
+    times): FETCH_SIZE, then WRITE_SIZE (KiB per dispatch), each scaled by
+    the factor that makes it read 1 GiB for that copy (HIP's device-to-device
+    copy kernel, __amd_rocclr_copyBuffer); a third run reads TCC_HIT_sum and
+    TCC_MISS_sum, the L2 hit rate of the search's requests."""
     import csv
     import shutil
     import subprocess
@@ -932,36 +943,47 @@ def measure_traffic(args):
              "--queries", str(min(args.queries, 250_000_000))]
     out = {"kernel": kern, "counters": {}}
     env = dict(os.environ, TMPDIR=tempfile.gettempdir())
-    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
-        d = tempfile.mkdtemp(prefix=f"sbwt_pmc_{ctr}_")
-        cmd = [prof, "--pmc", ctr, "-f", "csv", "-d", d, "-o", "run", "--", *child]
+    for ctrs in (("FETCH_SIZE",), ("WRITE_SIZE",), ("TCC_HIT_sum", "TCC_MISS_sum")):
+        tag = ctrs[0]
+        d = tempfile.mkdtemp(prefix=f"sbwt_pmc_{tag}_")
+        cmd = [prof, "--pmc", *ctrs, "-f", "csv", "-d", d, "-o", "run", "--", *child]
         log(f"traffic: {' '.join(cmd[:8])} ... (child run)")
         try:
             r = subprocess.run(cmd, capture_output=True, text=True, timeout=300, env=env, cwd=tempfile.gettempdir())
         except subprocess.TimeoutExpired:
-            return {"error": f"{ctr} run timed out"}
+            return {"error": f"{tag} run timed out"}
         files = [os.path.join(dp, f) for dp, _, fs in os.walk(d) for f in fs if f.endswith("counter_collection.csv")]
         if not files:
-            return {"error": f"{ctr} run produced no counters (rc {r.returncode}): {r.stderr[-300:]}"}
-        copy, search = [], []
+            return {"error": f"{tag} run produced no counters (rc {r.returncode}): {r.stderr[-300:]}"}
+        copy, search = {c: [] for c in ctrs}, {c: [] for c in ctrs}
         with open(files[0]) as fh:
             for row in csv.DictReader(fh):
-                name, v = row["Kernel_Name"], float(row["Counter_Value"])
+                name, c, v = row["Kernel_Name"], row.get("Counter_Name", tag), float(row["Counter_Value"])
+                if c not in search:
+                    continue
                 if "__amd_rocclr_copyBuffer" in name:
-                    copy.append(v)
+                    copy[c].append(v)
                 elif kern in name and ", true>" not in name.split("(")[0][-8:]:
-                    search.append(v)
+                    search[c].append(v)
         shutil.rmtree(d, ignore_errors=True)
-        if not copy or not search:
-            return {"error": f"{ctr}: copy or search kernel missing from the counters"}
+        if not all(search.values()):
+            return {"error": f"{tag}: search kernel missing from the counters"}
+        if len(ctrs) == 2:  # the L2 hit rate: a ratio, no calibration
+            hit = sum(search[ctrs[0]]) / len(search[ctrs[0]])
+            miss = sum(search[ctrs[1]]) / len(search[ctrs[1]])
+            out["tcc"] = {"hit_per_dispatch": round(hit), "miss_per_dispatch": round(miss),
+                          "hit_rate": round(hit / max(hit + miss, 1.0), 4)}
+            continue
+        if not copy[tag]:
+            return {"error": f"{tag}: copy kernel missing from the counters"}
         # the three 1 GiB copies are the largest copy dispatches of the child
-        big = sorted(copy)[-3:]
+        big = sorted(copy[tag])[-3:]
         copy_kib = sum(big) / len(big)
         if copy_kib < 100_000:
-            return {"error": f"{ctr}: the 1 GiB calibration copies were not found"}
+            return {"error": f"{tag}: the 1 GiB calibration copies were not found"}
         factor = (1 << 20) / copy_kib
-        kib = sum(search) / len(search)
-        out["counters"][ctr] = {"search_KiB_per_dispatch": round(kib, 1), "copy_KiB": round(copy_kib, 1),
+        kib = sum(search[tag]) / len(search[tag])
+        out["counters"][tag] = {"search_KiB_per_dispatch": round(kib, 1), "copy_KiB": round(copy_kib, 1),
                                 "scale": round(factor, 4), "bytes": round(kib * 1024 * factor)}
     out["queries_per_launch"] = min(args.queries, 250_000_000) if args.workload == "large" else None
     return out
