@@ -1,4 +1,4 @@
-// Answer stores of the read-search kernels.
+// Answer stores and prefilter flags of the read-search kernels.
 //
 // A lane answers the k-mers of its own rows in order, 8 bytes at a time. With
 // the index reads streaming through the L2, a partially written output line
@@ -7,7 +7,9 @@
 // collects its answers per aligned group of four output slots (32 bytes) in
 // LDS, slot-major (slot s of thread t at obuf[s * kAnswerStride + t], so a
 // wave's accesses never conflict), and a full group goes out as two 16-byte
-// stores; partial groups (row and item edges) go out slot by slot.
+// stores; partial groups (row and item edges) go out slot by slot. Runs of
+// one value (k-mers answered ~0 without a search) skip the buffer for their
+// whole groups.
 #pragma once
 #include "common.hpp"
 
@@ -39,18 +41,86 @@ struct AnswerBuffer {
     mask = 0;
   }
 
-  // out[q] = v; `last` flushes (the lane's item ends at q). vec: out is
-  // 16-byte aligned, so a group of four is two aligned 16-byte stores.
+  // out[q] = v; `last`: the lane's item ends at q. A lane answers its items'
+  // k-mers in order, so the buffer only ever holds the group of q - 1 (and is
+  // empty at a group start): a group goes out when its slot 3 is written or
+  // its item ends.
   __device__ __forceinline__ void put(u64* __restrict__ out, bool vec, u64 q, u64 v, bool last) {
-    const u64 g = q >> 2;
     const u32 sl = static_cast<u32>(q) & 3u;
-    if (g != group) {
-      flush(out, vec);
-      group = g;
-    }
     obuf[sl * kAnswerStride] = v;
     mask |= 1u << sl;
-    if (mask == 15 || last) flush(out, vec);
+    group = q >> 2;
+    if (sl == 3 || last) flush(out, vec);
+  }
+
+  // out[q .. q+n-1] = v (n >= 1); `last`: the lane's item ends at q+n-1.
+  // Whole aligned groups go straight from registers to memory.
+  __device__ __forceinline__ void put_run(u64* __restrict__ out, bool vec, u64 q, u64 n, u64 v, bool last) {
+    do {
+      const u32 sl = static_cast<u32>(q) & 3u;
+      if (sl == 0 && n >= 4) {
+        u64* o = out + q;
+        if (vec) {
+          const uint4 vv = make_uint4(static_cast<u32>(v), static_cast<u32>(v >> 32), static_cast<u32>(v),
+                                      static_cast<u32>(v >> 32));
+          reinterpret_cast<uint4*>(o)[0] = vv;
+          reinterpret_cast<uint4*>(o)[1] = vv;
+        } else {
+          o[0] = v;
+          o[1] = v;
+          o[2] = v;
+          o[3] = v;
+        }
+        q += 4;
+        n -= 4;
+      } else {
+        obuf[sl * kAnswerStride] = v;
+        mask |= 1u << sl;
+        group = q >> 2;
+        q++;
+        n--;
+        if (sl == 3 || (last && n == 0)) flush(out, vec);
+      }
+    } while (n);
+  }
+};
+
+// Prefilter flags (one byte per k-mer, nonzero = search it) read by a lane
+// walking its own k-mers in order. Thousands of lanes per CU each walk their
+// own region, so a flag read one byte at a time misses every cache and
+// fetches a whole 128-byte line from HBM per k-mer; instead a lane reads the
+// aligned 16-byte block holding the flag and keeps it as 16 bits.
+struct MaybeFlags {
+  u32 bits = 0;  // bit i: byte i of the cached block is nonzero
+  u32 tag = 0;   // the cached block's address >> 4 (low 32 bits)
+
+  __device__ __forceinline__ static u32 nonzero4(u32 x) {
+    x |= x >> 4;
+    x |= x >> 2;
+    x |= x >> 1;  // bit 8i: byte i is nonzero
+    return ((x & 0x01010101u) * 0x00204081u) >> 21 & 15u;
+  }
+
+  // Before a walk that starts at flag idx and only moves forward: marks the
+  // block before idx's as cached, which the walk never asks for.
+  __device__ __forceinline__ void reset(const u8* maybe, u64 idx) {
+    tag = static_cast<u32>(reinterpret_cast<uintptr_t>(maybe + idx) >> 4) - 1u;
+  }
+
+  // The number of zero flags from idx on, up to the end of idx's 16-byte
+  // block (0: maybe[idx] != 0), for a flag array of `total` bytes.
+  __device__ __forceinline__ u32 zeros(const u8* __restrict__ maybe, u64 total, u64 idx) {
+    const uintptr_t a = reinterpret_cast<uintptr_t>(maybe + idx);
+    const u32 t = static_cast<u32>(a >> 4), o = static_cast<u32>(a & 15);
+    if (t != tag) {
+      const uintptr_t blk = a - o;
+      // a block reaching outside the array (its first or last 15 bytes) is not cached
+      if (blk < reinterpret_cast<uintptr_t>(maybe) || idx - o + 16 > total) return maybe[idx] ? 0u : 1u;
+      const uint4 v = *reinterpret_cast<const uint4*>(blk);
+      bits = nonzero4(v.x) | nonzero4(v.y) << 4 | nonzero4(v.z) << 8 | nonzero4(v.w) << 12;
+      tag = t;
+    }
+    return static_cast<u32>(__builtin_ctz((bits >> o) | (0x10000u >> o)));
   }
 };
 

@@ -229,7 +229,9 @@ __global__ __launch_bounds__(256) void k_search_reads(DevIndex ix, const u64* __
     const u64 cnt = (rowk - j0) < seg ? (rowk - j0) : seg;
     const u64 rs = row_starts[r];
     const u64 obase = offsets[r];
-    const u8* mb = maybe ? maybe + offsets[r] : nullptr;
+    const u64 nk = maybe ? offsets[rows] : 0;
+    MaybeFlags mf;
+    if (maybe) mf.reset(maybe, obase + j0);
     const u32 k = ix.k;
     bool prev = false;
     u32 pnode = 0;
@@ -238,17 +240,31 @@ __global__ __launch_bounds__(256) void k_search_reads(DevIndex ix, const u64* __
     u32 knode = 0;
     Rolling<2> code{packed};
     Rolling<1> bad{nmask};
-    for (u64 j = j0; j < j0 + cnt; j++) {
+    const u64 jend = j0 + cnt;
+    for (u64 j = j0; j < jend; j++) {
+      // a run of k-mers answered ~0 without a search: certified absent,
+      // holding a non-ACGT character (every k-mer from j up to the first one
+      // holds it), or filtered out
+      u64 run = 0;
+      if (CERT && j >= clo && j <= chi) {
+        run = chi + 1 - j;
+      } else {
+        const u64 nb = NBREAK ? bad.at(rs + j) : 0;
+        const u32 p = nb ? static_cast<u32>(__clzll(nb)) : 64u;
+        if (p < k) run = p + 1;
+        else if (maybe) run = mf.zeros(maybe, nk, obase + j);
+      }
+      if (run) {
+        if (run > jend - j) run = jend - j;
+        ab.put_run(out, vec_out != 0, obase + j, run, kNone, j + run == jend);
+        prev = false;
+        j += run - 1;
+        continue;
+      }
       u64 res;
       u64 d = 0;  // end of a dead string found at this k-mer (0: none)
       const u64 win = code.at(rs + j);
-      if (CERT && j >= clo && j <= chi) {
-        res = kNone;
-      } else if (NBREAK && (bad.at(rs + j) >> (64 - k)) != 0) {
-        res = kNone;
-      } else if (mb && !mb[j]) {
-        res = kNone;
-      } else if (CERT && j == kpos) {
+      if (CERT && j == kpos) {
         res = knode;
       } else if (STREAM && prev) {
         res = stream_step<CB, COUNT>(ix, pnode, static_cast<u32>(win >> (64 - 2 * k)) & 3u, t);
@@ -260,7 +276,7 @@ __global__ __launch_bounds__(256) void k_search_reads(DevIndex ix, const u64* __
       }
       prev = res != kNone;
       pnode = static_cast<u32>(res);
-      ab.put(out, vec_out != 0, obase + j, res, j + 1 == j0 + cnt);
+      ab.put(out, vec_out != 0, obase + j, res, j + 1 == jend);
       if (CERT && d > j && d < rowk && !(clo <= chi && chi > j)) {
         u32 lp;
         const u64 rp = search_len<CB, COUNT>(ix, extract2(packed, rs + d), k, t, lp);

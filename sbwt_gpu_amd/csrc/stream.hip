@@ -94,6 +94,7 @@ struct Walker {
   u32 clo = 1, chi = 0, kpos = kNoPos, knode = 0, cp = 0;
   u32 nN = kNoPos;  // first non-ACGT position (row-relative) at or after j, if >= j; kNoPos = unknown
   u64 cw = 0, cw_idx = ~0ull;  // packed word holding the last streamed character
+  MaybeFlags mf;               // prefilter flags of the item's k-mers
   // One step's reads: chunks at columns a and b of one row, or a presearch
   // entry. Only the addresses live across the loads; update() recomputes the
   // rest from the walker's state.
@@ -141,6 +142,7 @@ struct Walker {
     chi = 0;
     kpos = kNoPos;
     nN = kNoPos;
+    if (rd.maybe) mf.reset(rd.maybe, obase + j);
   }
 
   __device__ __forceinline__ void start_search(const DevIndex& ix, const ReadsDev& rd, UTally& t, u32 pur,
@@ -163,6 +165,14 @@ struct Walker {
     ab.put(out, rd.vec_out != 0, obase + j, v, j + 1 >= jend);
     j++;
     if (COUNT) t.kmers++;
+  }
+
+  // v for the n k-mers from j on; moves to j + n
+  __device__ __forceinline__ void answer_run(u64* __restrict__ out, const ReadsDev& rd, AnswerBuffer& ab,
+                                             UTally& t, u32 n, u64 v) {
+    ab.put_run(out, rd.vec_out != 0, obase + j, n, v, j + n >= jend);
+    j += n;
+    if (COUNT) t.kmers += n;
   }
 
   // a k-mer before j was found absent with a dead string ending at char d
@@ -196,39 +206,34 @@ struct Walker {
   }
 
   // Answer what needs no index access, then set up one step.
-  __device__ __forceinline__ void advance(const DevIndex& ix, const ReadsDev& rd, u64* __restrict__ out,
+  // (nk: the number of k-mers of the batch, the length of rd.maybe)
+  __device__ __forceinline__ void advance(const DevIndex& ix, const ReadsDev& rd, u64 nk, u64* __restrict__ out,
                                           AnswerBuffer& ab, UTally& t) {
     const u32 k = ix.k;
     while (mode == M_IDLE && j < jend) {
-      // one k-mer per pass: answered here (certified, non-ACGT, filtered out,
-      // or already found by a certificate search), or handed to a step
-      bool emit = true;
-      u64 v = kNone;
-      if (j >= clo && j <= chi) {
-        prev = false;
-      } else if (rd.nbreak && has_n(rd, k)) {
-        prev = false;
-      } else if (rd.maybe && !rd.maybe[obase + j]) {
-        prev = false;
-      } else if (j == kpos) {
-        prev = true;
+      // a run of k-mers answered ~0 without a search: certified absent,
+      // holding a non-ACGT character (every k-mer from j up to that
+      // character holds it), or filtered out
+      u32 run = 0;
+      if (j >= clo && j <= chi) run = chi + 1 - j;
+      else if (rd.nbreak && has_n(rd, k)) run = nN + 1 - j;
+      else if (rd.maybe) run = mf.zeros(rd.maybe, nk, obase + j);
+      const bool known = j == kpos;  // found by a certificate search
+      if (run || known) {
+        prev = !run;
         pnode = knode;
-        v = knode;
-      } else {
-        emit = false;
-        if (rd.stream && prev) {
-          const u64 p = cbase + j + k - 1;  // the new character
-          if ((p >> 5) != cw_idx) {
-            cw_idx = p >> 5;
-            cw = rd.packed[cw_idx];
-          }
-          sc = static_cast<u32>(cw >> (62 - 2 * (p & 31))) & 3u;
-          mode = M_STREAM;
-        } else {
-          start_search(ix, rd, t, P_KMER, j, k);
+        answer_run(out, rd, ab, t, run ? (run < jend - j ? run : jend - j) : 1u, run ? kNone : knode);
+      } else if (rd.stream && prev) {
+        const u64 p = cbase + j + k - 1;  // the new character
+        if ((p >> 5) != cw_idx) {
+          cw_idx = p >> 5;
+          cw = rd.packed[cw_idx];
         }
+        sc = static_cast<u32>(cw >> (62 - 2 * (p & 31))) & 3u;
+        mode = M_STREAM;
+      } else {
+        start_search(ix, rd, t, P_KMER, j, k);
       }
-      if (emit) answer(out, rd, ab, t, v);
     }
   }
 
@@ -403,6 +408,7 @@ __global__ __launch_bounds__(256) void k_reads_uniform(DevIndex ix, ReadsDev rd,
   AnswerBuffer ab{obuf + threadIdx.x};
   const u32 lane = threadIdx.x & 63;
   const u64 total_items = rd.rows ? rd.work[rd.rows] : 0;
+  const u64 nk = rd.maybe && rd.rows ? rd.offsets[rd.rows] : 0;
   W w[P];
   UTally t;
   ItemPool pool;
@@ -411,7 +417,7 @@ __global__ __launch_bounds__(256) void k_reads_uniform(DevIndex ix, ReadsDev rd,
 #pragma unroll
     for (int p = 0; p < P; p++) {
       refill(w[p], pool, item_ctr, total_items, rd, lane);
-      w[p].advance(ix, rd, out, ab, t);
+      w[p].advance(ix, rd, nk, out, ab, t);
       active = active || w[p].mode != M_IDLE || w[p].live;
     }
     if (!__any(active)) break;

@@ -236,10 +236,20 @@ def test_masked_search_equals_plain_where_maybe(variant):
     reads = _reads(rng, g, 300, 150) + _reads(rng, _genome(rng, 50_000), 300, 150) + ["ACG", g[:31]]
     batch = sg.ReadBatch.from_host(reads, k)
     plain = sg.to_u64(ix.search_reads(batch, **variant))
-    mask = (rng.random(plain.size) < 0.6).astype(np.uint8)
-    mt = torch.from_numpy(mask).cuda()
-    got = sg.to_u64(ix.search_reads(batch, maybe=mt, **variant))
-    np.testing.assert_array_equal(got, np.where(mask == 1, plain, NONE))
+    # any nonzero byte means "maybe"; the kernels read flags in aligned
+    # 16-byte blocks, so also pass flags at every offset from a 16-byte
+    # boundary, and in runs that span blocks
+    vals = np.array([1, 2, 128, 255], np.uint8)
+    mask = np.where(rng.random(plain.size) < 0.6, vals[rng.integers(0, 4, plain.size)], 0).astype(np.uint8)
+    runs = np.repeat(rng.random(plain.size // 20 + 1) < 0.3, 20)[: plain.size].astype(np.uint8)
+    for m in (mask, runs):
+        want = np.where(m != 0, plain, NONE)
+        for shift in (0, 1, 7, 15):
+            store = torch.zeros(m.size + 32, dtype=torch.uint8, device="cuda")
+            mt = store[shift: shift + m.size]
+            mt.copy_(torch.from_numpy(m))
+            got = sg.to_u64(ix.search_reads(batch, maybe=mt, **variant))
+            np.testing.assert_array_equal(got, want, err_msg=f"flag offset {shift}")
     ones = torch.ones(plain.size, dtype=torch.uint8, device="cuda")
     np.testing.assert_array_equal(sg.to_u64(ix.search_reads(batch, maybe=ones, **variant)), plain)
 
