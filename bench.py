@@ -566,9 +566,13 @@ def run_prefilter(args) -> int:
     packed = torch.empty(L.sbwt_packed_words(n_chars), dtype=torch.int64, device=dev)
     nmask = torch.empty(L.sbwt_nmask_words(n_chars), dtype=torch.int64, device=dev)
     offsets = torch.empty(rows + 1, dtype=torch.int64, device=dev)
-    work = torch.empty(rows + 1, dtype=torch.int64, device=dev)
-    bound = L.sbwt_reads_work_bound(rows, total, args.seg)
-    items = torch.empty(bound, dtype=torch.int32, device=dev)
+    # work decompositions for the segment lengths tried per path
+    segs = sorted({args.seg, 32, 64})
+    work_of = {}
+    for sl in segs:
+        b = L.sbwt_reads_work_bound(rows, total, sl)
+        work_of[sl] = (torch.empty(rows + 1, dtype=torch.int64, device=dev),
+                       torch.empty(b, dtype=torch.int32, device=dev), b)
     pos = torch.empty(total, dtype=torch.int64, device=dev)
     kmers = torch.empty(total, dtype=torch.int64, device=dev)
     maybe = torch.empty(total, dtype=torch.uint8, device=dev)
@@ -577,24 +581,26 @@ def run_prefilter(args) -> int:
     stream = torch.cuda.current_stream()
     nsteps = max(args.steps, 1)
 
-    def prep():
+    def prep(seg):
+        work, items, _ = work_of[seg]
         check(L.sbwt_pack(_ptr(rchars), n_chars, _ptr(packed), _ptr(nmask), s))
         check(L.sbwt_kmer_offsets(_ptr(rstarts), rows, k, _ptr(offsets), s))
-        check(L.sbwt_reads_work(_ptr(offsets), rows, args.seg, _ptr(work), _ptr(items), s))
+        check(L.sbwt_reads_work(_ptr(offsets), rows, seg, _ptr(work), _ptr(items), s))
 
-    def search(flags, masked):
+    def search(flags, masked, seg):
+        work, items, bound = work_of[seg]
         a = (ix._h, _ptr(packed), _ptr(nmask), _ptr(rstarts), _ptr(offsets), _ptr(work), _ptr(items), rows, bound,
-             args.seg)
+             seg)
         if masked:
             check(L.sbwt_search_reads_masked(*a, _ptr(maybe), _ptr(out), flags, s))
         else:
             check(L.sbwt_search_reads(*a, _ptr(out), flags, s))
 
-    def make_path(flags, filt):
+    def make_path(flags, filt, seg):
         """filt: None (plain), "learned" (MLP + Bloom over its false
         negatives) or "bloom" (Bloom over every indexed k-mer)."""
         def run(ev=None):
-            prep()
+            prep(seg)
             if filt is not None:
                 check(L.sbwt_positions(_ptr(rstarts), _ptr(offsets), rows, k, _ptr(pos), s))
                 check(L.sbwt_extract_kmers(_ptr(packed), _ptr(pos), total, _ptr(kmers), s))
@@ -606,19 +612,50 @@ def run_prefilter(args) -> int:
                     check(L.sbwt_bloom_query(bloom_all._h, _ptr(kmers), total, _ptr(maybe), s))
             if ev:
                 ev[1].record(stream)
-            search(flags, filt is not None)
+            search(flags, filt is not None, seg)
             if ev:
                 ev[2].record(stream)
         return run
 
-    paths = {
-        "plain": make_path(0, None),
-        "plain_independent": make_path(_capi.READS_INDEPENDENT, None),
-        "filtered": make_path(0, "learned"),
-        "filtered_independent": make_path(_capi.READS_INDEPENDENT, "learned"),
-        "bloom_only": make_path(0, "bloom"),
-        "bloom_only_independent": make_path(_capi.READS_INDEPENDENT, "bloom"),
+    def tuned(flags, filt):
+        """Each path at its fastest segment length and kernel on this batch
+        (untimed trials; every trial must give the same answers)."""
+        kern = ((0, "uniform"), (_capi.READS_SEGMENT_LANES, "lanes")) if not flags & _capi.READS_INDEPENDENT else \
+            ((0, "lanes"), (_capi.READS_PERSISTENT, "uniform"))
+        best, ref = None, None
+        for sl in segs:
+            for kf, kname in kern:
+                run = make_path(flags | kf, filt, sl)
+                run()
+                ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+                ev[0].record(stream)
+                run()
+                run()
+                ev[1].record(stream)
+                torch.cuda.synchronize()
+                ms = ev[0].elapsed_time(ev[1]) / 2
+                if ref is None:
+                    ref = out.clone()
+                elif not torch.equal(out, ref):
+                    raise RuntimeError(f"answers differ at segment {sl}, kernel {kname}")
+                if best is None or ms < best[0]:
+                    best = (ms, sl, kname, run)
+        return best
+
+    specs = {
+        "plain": (0, None),
+        "plain_independent": (_capi.READS_INDEPENDENT, None),
+        "filtered": (0, "learned"),
+        "filtered_independent": (_capi.READS_INDEPENDENT, "learned"),
+        "bloom_only": (0, "bloom"),
+        "bloom_only_independent": (_capi.READS_INDEPENDENT, "bloom"),
     }
+    paths, chosen = {}, {}
+    for name, (fl, filt) in specs.items():
+        ms, sl, kname, run = tuned(fl, filt)
+        paths[name] = run
+        chosen[name] = {"segment_kmers": sl, "kernel": kname}
+    log(f"per-path segment length and kernel: {chosen}")
     results, answers = {}, {}
     for name, run in paths.items():
         for _ in range(args.warmup):
@@ -637,7 +674,7 @@ def run_prefilter(args) -> int:
         mrate = float(maybe.float().mean().item()) if "plain" not in name else None
         results[name] = {"kmers_per_s": round(total / el, 1), "ms_per_step": round(el * 1e3, 3),
                          "filter_kernel_ms": round(filt_ms, 3), "search_kernel_ms": round(search_ms, 3),
-                         "maybe_fraction": round(mrate, 4) if mrate is not None else None}
+                         "maybe_fraction": round(mrate, 4) if mrate is not None else None, **chosen[name]}
         log(f"{name}: {total / el / 1e9:.3f} G k-mers/s ({el * 1e3:.2f} ms/step; filter {filt_ms:.2f} ms, "
             f"search {search_ms:.2f} ms, maybe {mrate})")
     plain = answers["plain"]
@@ -657,7 +694,9 @@ def run_prefilter(args) -> int:
     pf_s = fr["filter_kernel_ms"] / 1e3
     ix.counters(reset=True)
     pf.maybe(kmers, out=maybe)
-    search(_capi.COUNT, True)
+    fk = _capi.READS_SEGMENT_LANES if chosen["filtered"]["kernel"] == "lanes" else 0
+    prep(chosen["filtered"]["segment_kmers"])
+    search(_capi.COUNT | fk, True, chosen["filtered"]["segment_kmers"])
     torch.cuda.synchronize()
     ctr = ix.counters(reset=True)
     search_gbps = int(ctr["lines"]) * 128 / (fr["search_kernel_ms"] / 1e3) / 1e9
@@ -667,7 +706,8 @@ def run_prefilter(args) -> int:
                 "kernel_avg_ms": fr["filter_kernel_ms"], "flops_per_kmer": MLP_FLOPS_PER_KMER}
     search_roof = {"bound": "hbm", "achieved": round(search_gbps, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                    "frac": round(search_gbps / HBM_PEAK_GBPS, 4), "traffic": None,
-                   "kernel": "k_search_reads (masked)", "kernel_avg_ms": fr["search_kernel_ms"],
+                   "kernel": ("k_search_reads" if fk else "k_reads_uniform") + " (masked)",
+                   "kernel_avg_ms": fr["search_kernel_ms"],
                    "lines_per_launch": int(ctr["lines"])}
     roofline = dict(mlp_roof if fr["filter_kernel_ms"] >= fr["search_kernel_ms"] else search_roof)
     roofline["other_kernel"] = search_roof if roofline["kernel"] == "k_prefilter" else mlp_roof
