@@ -116,7 +116,7 @@ struct MaybeFlags {
       const uintptr_t blk = a - o;
       // a block reaching outside the array (its first or last 15 bytes) is not cached
       if (blk < reinterpret_cast<uintptr_t>(maybe) || idx - o + 16 > total) return maybe[idx] ? 0u : 1u;
-      const uint4 v = *reinterpret_cast<const uint4*>(blk);
+      const uint4 v = *reinterpret_cast<const uint4*>(maybe + (idx - o));  // (a global load, not flat)
       bits = nonzero4(v.x) | nonzero4(v.y) << 4 | nonzero4(v.z) << 8 | nonzero4(v.w) << 12;
       tag = t;
     }

@@ -13,43 +13,67 @@
 
 namespace sbwt {
 
-// The reference packing table (create_kmer_string.hpp:20): C/c->1, G/g->2,
-// T/t->3, every other byte -> 0; `bad` marks bytes that are not A/C/G/T.
-__device__ __forceinline__ u32 code_of(u32 b, u32& bad) {
-  const u32 lc = b | 0x20u;
-  bad = (lc == 'a' || lc == 'c' || lc == 'g' || lc == 't') ? 0u : 1u;
-  return lc == 'c' ? 1u : (lc == 'g' ? 2u : (lc == 't' ? 3u : 0u));
+// The reference packing table (create_kmer_string.hpp:20) on four bytes at
+// once (byte 0 first): C/c->1, G/g->2, T/t->3, every other byte -> 0.
+// Returns the four 2-bit codes MSB-first in 8 bits and sets `bad` to the four
+// non-ACGT flags MSB-first in 4 bits. Branch-free: a chain of per-byte
+// selects compiles to a branch per character.
+__device__ __forceinline__ u32 code4(u32 w, u32& bad) {
+  const u32 lc = w | 0x20202020u;  // lower case
+  // 0x80 in each byte equal to a, c, g or t (an exact zero-byte test of lc ^ v)
+  u32 ok = 0;
+  const u32 letters[4] = {0x61u, 0x63u, 0x67u, 0x74u};
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const u32 y = lc ^ (letters[i] * 0x01010101u);
+    ok |= ~(((y & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | y) & 0x80808080u;
+  }
+  // bits 2:1 of a, c, g, t are 00, 01, 11, 10: a Gray code of 0, 1, 2, 3
+  const u32 x = (lc >> 1) & 0x03030303u;
+  const u32 code = (x ^ ((x >> 1) & 0x01010101u)) & ((ok >> 7) * 3u);
+  // gather byte i's field to bits 7-2i:6-2i (codes) and bit 3-i (flags); the
+  // products' partial terms never overlap, so nothing carries
+  bad = ((((~ok & 0x80808080u) >> 7) * 0x08040201u) >> 24) & 0xFu;
+  return (code * 0x40100401u) >> 24;
 }
 
 // Thread q packs characters [32q, 32q+32): one packed word (char 0 in bits
 // 63:62) and one 32-bit half of the mask (MSB-first, so the half of mask
-// word q/2 that holds them is u32 q^1 in little-endian memory).
+// word q/2 that holds them is u32 q^1 in little-endian memory). Characters
+// past n read as byte 0 (not ACGT).
 __global__ __launch_bounds__(256) void k_pack(const u8* __restrict__ chars, u64 n, u64* __restrict__ packed,
                                               u64 pwords, u32* __restrict__ mask32, u64 mhalves, u64 threads) {
   const u64 q = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x;
   if (q >= threads) return;
   const u64 first = q * 32;
-  u32 b[32];
+  u32 w[8];
   if (first + 32 <= n && (reinterpret_cast<uintptr_t>(chars + first) & 15) == 0) {
     const uint4* p = reinterpret_cast<const uint4*>(chars + first);
     const uint4 v0 = p[0], v1 = p[1];
-    const u32 w[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
-#pragma unroll
-    for (int t = 0; t < 32; t++) b[t] = (w[t >> 2] >> (8 * (t & 3))) & 0xffu;
+    w[0] = v0.x; w[1] = v0.y; w[2] = v0.z; w[3] = v0.w;
+    w[4] = v1.x; w[5] = v1.y; w[6] = v1.z; w[7] = v1.w;
   } else {
 #pragma unroll
-    for (int t = 0; t < 32; t++) b[t] = (first + t < n) ? chars[first + t] : 0u;
-  }
-  u64 word = 0;
-  u32 half = 0;
+    for (int i = 0; i < 8; i++) {
+      u32 v = 0;
 #pragma unroll
-  for (int t = 0; t < 32; t++) {
-    u32 bad;
-    const u32 c = code_of(b[t], bad);
-    word |= static_cast<u64>(c) << (62 - 2 * t);
-    half |= bad << (31 - t);
+      for (int t = 0; t < 4; t++) {
+        const u64 at = first + 4 * i + t;
+        v |= (at < n ? static_cast<u32>(chars[at]) : 0u) << (8 * t);
+      }
+      w[i] = v;
+    }
   }
-  if (q < pwords) packed[q] = word;
+  u32 hi = 0, lo = 0, half = 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    u32 b0, b1;
+    hi |= code4(w[i], b0) << (24 - 8 * i);
+    lo |= code4(w[i + 4], b1) << (24 - 8 * i);
+    half |= b0 << (28 - 4 * i);
+    half |= b1 << (12 - 4 * i);
+  }
+  if (q < pwords) packed[q] = (static_cast<u64>(hi) << 32) | lo;
   if (mask32 && (q ^ 1) < mhalves) mask32[q ^ 1] = half;
 }
 
