@@ -566,8 +566,9 @@ def run_prefilter(args) -> int:
     packed = torch.empty(L.sbwt_packed_words(n_chars), dtype=torch.int64, device=dev)
     nmask = torch.empty(L.sbwt_nmask_words(n_chars), dtype=torch.int64, device=dev)
     offsets = torch.empty(rows + 1, dtype=torch.int64, device=dev)
-    # work decompositions for the segment lengths tried per path
-    segs = sorted({args.seg, 32, 64})
+    # work decompositions for the segment lengths tried per path (per-k-mer
+    # searches gain nothing from long segments, so they also try short ones)
+    segs = sorted({args.seg, 8, 16, 32, 64})
     work_of = {}
     for sl in segs:
         b = L.sbwt_reads_work_bound(rows, total, sl)
@@ -624,6 +625,8 @@ def run_prefilter(args) -> int:
             ((0, "lanes"), (_capi.READS_PERSISTENT, "uniform"))
         best, ref = None, None
         for sl in segs:
+            if sl < 32 and not flags & _capi.READS_INDEPENDENT:
+                continue
             for kf, kname in kern:
                 run = make_path(flags | kf, filt, sl)
                 run()
