@@ -602,15 +602,17 @@ def run_prefilter(args) -> int:
         negatives) or "bloom" (Bloom over every indexed k-mer)."""
         def run(ev=None):
             prep(seg)
-            if filt is not None:
+            if filt == "learned":  # the matrix-core kernel takes extracted k-mers
                 check(L.sbwt_positions(_ptr(rstarts), _ptr(offsets), rows, k, _ptr(pos), s))
                 check(L.sbwt_extract_kmers(_ptr(packed), _ptr(pos), total, _ptr(kmers), s))
                 if ev:
                     ev[0].record(stream)
-                if filt == "learned":
-                    pf.maybe(kmers, out=maybe)
-                else:
-                    check(L.sbwt_bloom_query(bloom_all._h, _ptr(kmers), total, _ptr(maybe), s))
+                pf.maybe(kmers, out=maybe)
+            elif filt == "bloom":  # straight from the packed reads
+                if ev:
+                    ev[0].record(stream)
+                check(L.sbwt_bloom_query_reads(bloom_all._h, _ptr(packed), _ptr(rstarts), _ptr(offsets), rows,
+                                               _ptr(maybe), s))
             if ev:
                 ev[1].record(stream)
             search(flags, filt is not None, seg)

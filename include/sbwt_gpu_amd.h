@@ -285,11 +285,13 @@ int sbwt_mlp_train_step(sbwt_mlp *nn, const uint64_t *d_kmers, const float *d_la
 int sbwt_mlp_logits(const sbwt_mlp *nn, const uint64_t *d_kmers, uint64_t count,
                     float *d_logits, int reference, void *stream);
 
-/* Register-blocked Bloom filter over k-mers (all bits of a key in one
- * 64-bit word), sized for `expected` keys at false-positive rate fpr. Keys
- * are top-aligned k-mers; only their top 2k bits are hashed, so
- * 32-character windows (sbwt_extract_kmers) and k-mers with zeroed low bits
- * (sbwt_enumerate_kmers) are the same key. */
+/* Blocked Bloom filter over k-mers, sized for `expected` keys at
+ * false-positive rate fpr. A key's bits lie in one 16-byte pair of a
+ * 128-byte block, and the block is chosen by the key's minimizer (its
+ * 16-mer, or k-mer for k < 16, of smallest hash), so the consecutive k-mers
+ * of a read mostly share a cache line. Keys are top-aligned k-mers; only
+ * their top 2k bits are hashed, so 32-character windows (sbwt_extract_kmers)
+ * and k-mers with zeroed low bits (sbwt_enumerate_kmers) are the same key. */
 int sbwt_bloom_create(uint32_t k, uint64_t expected, double fpr, sbwt_bloom **out);
 int sbwt_bloom_free(sbwt_bloom *b);
 uint64_t sbwt_bloom_bytes(const sbwt_bloom *b);
@@ -299,6 +301,14 @@ int sbwt_bloom_get_words(const sbwt_bloom *b, uint64_t *words, uint64_t count);
 int sbwt_bloom_insert(sbwt_bloom *b, const uint64_t *d_keys, uint64_t count, void *stream);
 int sbwt_bloom_query(const sbwt_bloom *b, const uint64_t *d_keys, uint64_t count,
                      uint8_t *d_out, void *stream);
+/* The filter's answer for every k-mer of every row, read straight from the
+ * packed rows (sbwt_pack): d_out[d_offsets[r] + j] for the k-mer at
+ * character j of row r (d_offsets from sbwt_kmer_offsets for the filter's
+ * k). The same answers as sbwt_bloom_query on the rows' extracted k-mers,
+ * without materialising them. */
+int sbwt_bloom_query_reads(const sbwt_bloom *b, const uint64_t *d_packed,
+                           const uint64_t *d_row_starts, const uint64_t *d_offsets,
+                           uint64_t rows, uint8_t *d_out, void *stream);
 
 /* d_maybe[q] = 1 if logit(k-mer q) >= threshold or the Bloom filter (may be
  * NULL; built for the model's k) holds it; k-mers with 0 are absent from the

@@ -15,7 +15,10 @@ Rounding rules the GPU kernels follow and this oracle restates:
 
 Everything here is float64 numpy. Pinned in tests/test_oracle_mlp.py by
 central finite differences of the loss and by torch autograd over an
-independently written forward (dense one-hot matmul).
+independently written forward (dense one-hot matmul). The Bloom filter's
+restatement (below) is pinned there by splitmix64's published first output,
+128-bit products in Python integers, the minimizer recomputed from the
+k-mer's text, and its Poisson sizing model against simulation.
 """
 from __future__ import annotations
 
@@ -147,15 +150,26 @@ def he_params(k: int, seed: int) -> np.ndarray:
     return p
 
 
-# ---- Bloom filter: register-blocked, 64-bit words ----------------------------
-# The GPU filter's scheme, restated: word = high 64 bits of h * words with
-# h = mix64(key ^ golden); its `hashes` bits are bits 6i..6i+5 of mix64(h).
-# Sized by bloom_size: the smallest bits/key (grid of 1/20 of the unblocked
-# -ln p / ln^2 2) and hash count 1..10 whose Poisson-averaged false-positive
-# rate reaches p (Putze, Sanders, Singler 2007, register-blocked filters).
+# ---- Bloom filter: blocks chosen by the key's minimizer ----------------------
+# The GPU filter's scheme, restated (sbwt_gpu_amd/csrc/mlp.hip, bloom_locate):
+# a key's bits lie in one 16-byte pair of 64-bit words inside a 128-byte block
+# of 16 words. block = high 64 bits of mix64(minimizer ^ BLOCK_SEED) * blocks,
+# the minimizer being the key's m-mer code c (m = min(k, 16)) with the
+# smallest o(c) = x ^ (x >> 16), x = c * 0x9E3779B1 mod 2^32 (leftmost on
+# ties); h = mix64(key ^ golden), pair = h >> 61; bit i (i < hashes) = bits
+# 7i..7i+6 of mix64(h), over the pair's 128 bits (bits 0..63 in the pair's
+# first word). Sized by bloom_size: the smallest bits/key (grid of 1/20 of
+# the unblocked -ln p / ln^2 2) and hash count 1..9 whose Poisson-averaged
+# false-positive rate over 128-bit blocks reaches p (Putze, Sanders, Singler
+# 2007, register-blocked filters), times CLUMP_MARGIN for the keys that share
+# a minimizer and so land in one block together.
 
 M64 = (1 << 64) - 1
 GOLDEN = 0x9E3779B97F4A7C15
+BLOCK_SEED = 0x632BE59BD9B4E019
+BLOCK_WORDS = 16
+CLUMP_MARGIN = 1.2
+MAX_HASHES = 9
 
 
 def mix64(x: int) -> int:
@@ -191,11 +205,34 @@ def mulhi64_np(a: np.ndarray, b: int) -> np.ndarray:
     return p3 + (p1 >> np.uint64(32)) + (p2 >> np.uint64(32)) + (mid >> np.uint64(32))
 
 
+def minimizer_order(c: np.ndarray) -> np.ndarray:
+    """o(c) = x ^ (x >> 16), x = c * 0x9E3779B1 mod 2^32."""
+    x = (np.asarray(c, dtype=np.uint64) * np.uint64(0x9E3779B1)) & np.uint64(0xFFFFFFFF)
+    return x ^ (x >> np.uint64(16))
+
+
+def minimizer_np(keys: np.ndarray, k: int) -> np.ndarray:
+    """The minimizer code of every top-aligned key (leftmost on ties)."""
+    m = min(k, 16)
+    cmask = np.uint64((1 << (2 * m)) - 1)
+    keys = np.asarray(keys, dtype=np.uint64)
+    best = (keys >> np.uint64(64 - 2 * m)) & cmask
+    bo = minimizer_order(best)
+    for i in range(1, k - m + 1):
+        c = (keys >> np.uint64(64 - 2 * m - 2 * i)) & cmask
+        o = minimizer_order(c)
+        sel = o < bo
+        best = np.where(sel, c, best)
+        bo = np.where(sel, o, bo)
+    return best
+
+
 def bloom_fpr(bits_per_key: float, h: int) -> float:
+    """Poisson-averaged false-positive rate of 128-bit register blocks."""
     import math
 
-    lam = 64.0 / bits_per_key
-    per = math.log1p(-1.0 / 64.0) * h
+    lam = 128.0 / bits_per_key
+    per = math.log1p(-1.0 / 128.0) * h
     pmf, s = math.exp(-lam), 0.0
     nmax = int(lam + 12.0 * math.sqrt(lam) + 24.0)
     for n in range(nmax + 1):
@@ -204,13 +241,13 @@ def bloom_fpr(bits_per_key: float, h: int) -> float:
     return s
 
 
-def bloom_size(p: float):
-    """(bits per key, hashes) for false-positive rate p."""
+def bloom_model_size(p: float):
+    """(bits per key, hashes) at which the Poisson model reaches p."""
     import math
 
     base = -math.log(p) / (math.log(2) * math.log(2))
     best, hashes = 0.0, 1
-    for h in range(1, 11):
+    for h in range(1, MAX_HASHES + 1):
         for j in range(201):
             bpk = base * (1.0 + 0.05 * j)
             if bloom_fpr(bpk, h) <= p:
@@ -218,40 +255,55 @@ def bloom_size(p: float):
                     best, hashes = bpk, h
                 break
     if best == 0.0:
-        best, hashes = base * 11.0, 10
+        best, hashes = base * 11.0, MAX_HASHES
     return best, hashes
 
 
+def bloom_size(p: float):
+    """(bits per key, hashes) of a filter for false-positive rate p."""
+    bpk, h = bloom_model_size(p)
+    return bpk * CLUMP_MARGIN, h
+
+
 def bloom_geometry(expected: int, fpr: float):
-    """(64-bit words, hashes) of a filter for `expected` keys."""
+    """(128-byte blocks, hashes) of a filter for `expected` keys."""
     import math
 
     bpk, h = bloom_size(fpr)
-    return (int(math.ceil(expected * bpk / 64.0)) if expected else 0), h
+    return (int(math.ceil(expected * bpk / (64.0 * BLOCK_WORDS))) if expected else 0), h
 
 
 def key_mask(k: int) -> int:
     return M64 if k == 32 else (M64 ^ ((1 << (64 - 2 * k)) - 1))
 
 
-def bloom_locate(keys: np.ndarray, words: int, hashes: int, k: int):
-    """(word index, bit mask) of every key (numpy uint64 arrays)."""
+def bloom_locate(keys: np.ndarray, blocks: int, hashes: int, k: int):
+    """(index of the key's pair's first word, mask of that word, mask of the
+    next word) of every key (numpy uint64 arrays)."""
     key = np.asarray(keys, dtype=np.uint64) & np.uint64(key_mask(k))
+    blk = mulhi64_np(mix64_np(minimizer_np(key, k) ^ np.uint64(BLOCK_SEED)), blocks)
     h = mix64_np(key ^ np.uint64(GOLDEN))
     h2 = mix64_np(h)
-    mask = np.zeros_like(h)
+    lo = np.zeros_like(h)
+    hi = np.zeros_like(h)
     for i in range(hashes):
-        mask |= np.uint64(1) << ((h2 >> np.uint64(6 * i)) & np.uint64(63))
-    return mulhi64_np(h, words), mask
+        b = (h2 >> np.uint64(7 * i)) & np.uint64(127)
+        bit = np.uint64(1) << (b & np.uint64(63))
+        first = b < np.uint64(64)
+        lo |= np.where(first, bit, np.uint64(0))
+        hi |= np.where(first, np.uint64(0), bit)
+    return blk * np.uint64(BLOCK_WORDS) + (h >> np.uint64(61)) * np.uint64(2), lo, hi
 
 
-def bloom_build(keys: np.ndarray, words: int, hashes: int, k: int) -> np.ndarray:
-    w, m = bloom_locate(keys, words, hashes, k)
-    out = np.zeros(words, dtype=np.uint64)
-    np.bitwise_or.at(out, w.astype(np.int64), m)
+def bloom_build(keys: np.ndarray, blocks: int, hashes: int, k: int) -> np.ndarray:
+    w, lo, hi = bloom_locate(keys, blocks, hashes, k)
+    out = np.zeros(blocks * BLOCK_WORDS, dtype=np.uint64)
+    np.bitwise_or.at(out, w.astype(np.int64), lo)
+    np.bitwise_or.at(out, (w + np.uint64(1)).astype(np.int64), hi)
     return out
 
 
 def bloom_query(bits: np.ndarray, keys: np.ndarray, hashes: int, k: int) -> np.ndarray:
-    w, m = bloom_locate(keys, bits.size, hashes, k)
-    return (bits[w.astype(np.int64)] & m) == m
+    w, lo, hi = bloom_locate(keys, bits.size // BLOCK_WORDS, hashes, k)
+    wi = w.astype(np.int64)
+    return ((bits[wi] & lo) == lo) & ((bits[wi + 1] & hi) == hi)

@@ -79,6 +79,7 @@ SIGNATURES = {
     "sbwt_bloom_get_words": (i32, [vp, vp, u64]),
     "sbwt_bloom_insert": (i32, [vp, vp, u64, vp]),
     "sbwt_bloom_query": (i32, [vp, vp, u64, vp, vp]),
+    "sbwt_bloom_query_reads": (i32, [vp, vp, vp, vp, u64, vp, vp]),
     "sbwt_prefilter": (i32, [vp, C.c_float, vp, vp, u64, vp, vp]),
     "sbwt_gather_roof": (i32, [vp, u64, u32, u32, u32, u64, u64, vp, vp]),
     "sbwt_parse_lines": (i32, [cp, u64, vp, vp, u64, u64p]),

@@ -462,6 +462,17 @@ class Bloom:
         check(self.lib.sbwt_bloom_query(self._h, _ptr(keys), keys.numel(), _ptr(out), _stream()))
         return out
 
+    def query_reads(self, reads: "ReadBatch", out: torch.Tensor | None = None) -> torch.Tensor:
+        """The filter's answer for every k-mer of the batch (in output
+        order), read from the packed rows without extracting k-mers."""
+        if reads.k != self.k:
+            raise ValueError(f"reads prepared for k={reads.k}, filter built for k={self.k}")
+        if out is None:
+            out = torch.empty(max(reads.total_kmers, 1), dtype=torch.uint8, device=reads.chars.device)
+        check(self.lib.sbwt_bloom_query_reads(self._h, _ptr(reads.packed), _ptr(reads.row_starts),
+                                              _ptr(reads.offsets), reads.rows, _ptr(out), _stream()))
+        return out[: reads.total_kmers]
+
     def close(self):
         if self._h:
             self.lib.sbwt_bloom_free(self._h)

@@ -81,7 +81,7 @@ struct sbwt_mlp {
 };
 
 struct sbwt_bloom {
-  sbwt::u64 blocks = 0;  // 64-bit words, one per key's block
+  sbwt::u64 blocks = 0;  // 128-byte blocks (16 words each)
   sbwt::u32 hashes = 0;
   sbwt::u32 k = 32;
   sbwt::u64 keymask = ~0ull;  // the top 2k bits
@@ -554,10 +554,21 @@ static int mlp_grad(sbwt_mlp* nn, const u64* kmers, const float* y, u64 B, hipSt
   return SBWT_OK;
 }
 
-// ---- Bloom filter: register-blocked (Putze, Sanders, Singler 2007) --------
-// All `hashes` bits of a key fall in one 64-bit word: a lookup is one
-// 8-byte load and a mask compare. Word = fastrange(h, words) of
-// h = mix64(key ^ golden); bit i = bits 6i..6i+5 of mix64(h) (hashes <= 10).
+// ---- Bloom filter: blocks chosen by the key's minimizer ---------------------
+// A key's bits all lie in one 16-byte pair of 64-bit words (a register-blocked
+// filter, Putze, Sanders, Singler 2007, with 128-bit blocks) inside one
+// 128-byte block. The block is chosen by the key's minimizer, so the
+// consecutive k-mers of a read, which mostly share it, share one cache line;
+// the pair and the bits are chosen by the key itself:
+//   minimizer = the m-mer (m = min(k, 16)) of the key whose code c has the
+//               smallest o(c) = x ^ (x >> 16), x = c * 0x9E3779B1 mod 2^32
+//               (the leftmost one on ties)
+//   block     = fastrange(mix64(minimizer ^ kBloomBlockSeed), blocks)
+//   h         = mix64(key ^ golden); pair = h >> 61 (8 pairs per block)
+//   bit i     = bits 7i..7i+6 of mix64(h), over the pair's 128 bits (i < hashes <= 9)
+// A lookup is one 16-byte load and two mask compares.
+constexpr u32 kBloomBlockWords = 16;  // 128 bytes
+constexpr u64 kBloomBlockSeed = 0x632BE59BD9B4E019ull;
 
 __device__ __host__ __forceinline__ u64 mix64(u64 x) {
   x ^= x >> 30;
@@ -568,34 +579,87 @@ __device__ __host__ __forceinline__ u64 mix64(u64 x) {
   return x;
 }
 
-__device__ __forceinline__ u64 bloom_locate(u64 words, u32 hashes, u64 key, u64& mask) {
-  const u64 h = mix64(key ^ 0x9E3779B97F4A7C15ull);
-  u64 h2 = mix64(h);
-  mask = 0;
-  for (u32 i = 0; i < hashes; i++, h2 >>= 6) mask |= 1ull << (h2 & 63);
-  return static_cast<u64>((static_cast<unsigned __int128>(h) * words) >> 64);
+// the minimizer of a top-aligned key of k characters (2m-bit code)
+__device__ __forceinline__ u32 kmer_minimizer(u64 key, u32 k) {
+  const u32 m = k < 16 ? k : 16;
+  const u32 mbits = 2 * m;
+  const u32 cmask = mbits == 32 ? 0xFFFFFFFFu : ((1u << mbits) - 1u);
+  u32 best = static_cast<u32>(key >> (64 - mbits)) & cmask;
+  u32 x = best * 0x9E3779B1u;
+  u32 bo = x ^ (x >> 16);
+  for (u32 i = 1; i + m <= k; i++) {
+    const u32 c = static_cast<u32>(key >> (64 - mbits - 2 * i)) & cmask;
+    x = c * 0x9E3779B1u;
+    const u32 o = x ^ (x >> 16);
+    best = o < bo ? c : best;
+    bo = o < bo ? o : bo;
+  }
+  return best;
 }
 
-__global__ void k_bloom_insert(u64* __restrict__ bits, u64 words, u32 hashes, u64 keymask,
+// The index of the key's pair (its first word) and the pair's two masks.
+__device__ __forceinline__ u64 bloom_locate(u64 blocks, u32 hashes, u32 k, u64 key, u64& mlo, u64& mhi) {
+  const u64 hb = mix64(static_cast<u64>(kmer_minimizer(key, k)) ^ kBloomBlockSeed);
+  const u64 blk = static_cast<u64>((static_cast<unsigned __int128>(hb) * blocks) >> 64);
+  const u64 h = mix64(key ^ 0x9E3779B97F4A7C15ull);
+  u64 h2 = mix64(h);
+  mlo = 0;
+  mhi = 0;
+  for (u32 i = 0; i < hashes; i++, h2 >>= 7) {
+    const u32 b = static_cast<u32>(h2) & 127u;
+    const u64 bit = 1ull << (b & 63u);
+    mlo |= b < 64 ? bit : 0ull;
+    mhi |= b < 64 ? 0ull : bit;
+  }
+  return blk * kBloomBlockWords + (h >> 61) * 2;
+}
+
+__global__ void k_bloom_insert(u64* __restrict__ bits, u64 blocks, u32 hashes, u32 k, u64 keymask,
                                const u64* __restrict__ keys, u64 count) {
   const u64 q = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x;
   if (q >= count) return;
-  u64 mask;
-  const u64 w = bloom_locate(words, hashes, keys[q] & keymask, mask);
-  atomicOr(reinterpret_cast<unsigned long long*>(bits + w), mask);
+  u64 mlo, mhi;
+  const u64 w = bloom_locate(blocks, hashes, k, keys[q] & keymask, mlo, mhi);
+  if (mlo) atomicOr(reinterpret_cast<unsigned long long*>(bits + w), mlo);
+  if (mhi) atomicOr(reinterpret_cast<unsigned long long*>(bits + w + 1), mhi);
 }
 
-__device__ __forceinline__ bool bloom_has(const u64* __restrict__ bits, u64 words, u32 hashes, u64 key) {
-  u64 mask;
-  const u64 w = bloom_locate(words, hashes, key, mask);
-  return (bits[w] & mask) == mask;
+// the pair's 16 bytes hold both masks
+__device__ __forceinline__ bool bloom_match(uint4 v, u64 mlo, u64 mhi) {
+  const u64 lo = static_cast<u64>(v.x) | static_cast<u64>(v.y) << 32;
+  const u64 hi = static_cast<u64>(v.z) | static_cast<u64>(v.w) << 32;
+  return (lo & mlo) == mlo && (hi & mhi) == mhi;
 }
 
-__global__ void k_bloom_query(const u64* __restrict__ bits, u64 blocks, u32 hashes, u64 keymask,
+__device__ __forceinline__ bool bloom_has(const u64* __restrict__ bits, u64 blocks, u32 hashes, u32 k, u64 key) {
+  u64 mlo, mhi;
+  const u64 w = bloom_locate(blocks, hashes, k, key, mlo, mhi);
+  return bloom_match(*reinterpret_cast<const uint4*>(bits + w), mlo, mhi);
+}
+
+__global__ void k_bloom_query(const u64* __restrict__ bits, u64 blocks, u32 hashes, u32 k, u64 keymask,
                               const u64* __restrict__ keys, u64 count, u8* __restrict__ out) {
   const u64 q = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x;
   if (q >= count) return;
-  out[q] = bits && blocks ? (bloom_has(bits, blocks, hashes, keys[q] & keymask) ? 1 : 0) : 0;
+  out[q] = bits && blocks ? (bloom_has(bits, blocks, hashes, k, keys[q] & keymask) ? 1 : 0) : 0;
+}
+
+// out[offsets[r] + j] = the filter's answer for the k-mer at character j of
+// row r, read straight from the packed rows: one wave per row (grid-stride),
+// its lanes on consecutive k-mers, which mostly share a block.
+__global__ __launch_bounds__(64) void k_bloom_query_reads(const u64* __restrict__ bits, u64 blocks, u32 hashes,
+                                                          u32 k, u64 keymask, const u64* __restrict__ packed,
+                                                          const u64* __restrict__ row_starts,
+                                                          const u64* __restrict__ offsets, u64 rows,
+                                                          u8* __restrict__ out) {
+  for (u64 r = blockIdx.x; r < rows; r += gridDim.x) {
+    const u64 o0 = offsets[r], nk = offsets[r + 1] - o0, rs = row_starts[r];
+    for (u64 j = threadIdx.x; j < nk; j += 64) {
+      const u64 b = (rs + j) << 1;  // the window's first bit in the packed string
+      const u64 win = (packed[b >> 6] << (b & 63)) | ((packed[(b >> 6) + 1] >> (63 - (b & 63))) >> 1);
+      out[o0 + j] = bloom_has(bits, blocks, hashes, k, win & keymask) ? 1 : 0;
+    }
+  }
 }
 
 // maybe[q] = logit >= threshold || bloom(k-mer q); fused MLP + Bloom. Each
@@ -607,7 +671,7 @@ __global__ void k_bloom_query(const u64* __restrict__ bits, u64 blocks, u32 hash
 // it needs and never the ones still in flight.
 template <bool BLOOM>
 __global__ __launch_bounds__(256) void k_prefilter(MlpDev md, float threshold, const u64* __restrict__ bits,
-                                                   u64 blocks, u32 hashes, u64 keymask,
+                                                   u64 blocks, u32 hashes, u32 klen, u64 keymask,
                                                    const u64* __restrict__ kmers, u64 count,
                                                    u8* __restrict__ maybe) {
   __shared__ OneHotLds oh;
@@ -625,14 +689,18 @@ __global__ __launch_bounds__(256) void k_prefilter(MlpDev md, float threshold, c
     const u64 i = g * 64 + lane;
     return kmers[i < count ? i : count - 1];
   };
-  // Fetch the Bloom word of k-mer kc if the model rejects it (every lane
-  // loads: lanes with nothing to fetch read word 0).
-  auto fetch = [&](u64 kc, float zc, u64& v, u64& mask) {
-    const u64 wi = bloom_locate(blocks, hashes, kc & keymask, mask);
-    v = bits[zc >= threshold ? 0 : wi];
+  // Fetch the Bloom pair of k-mer kc if the model rejects it (every lane
+  // loads: lanes with nothing to fetch read pair 0).
+  struct Probe {
+    uint4 v;
+    u64 mlo, mhi;
   };
-  auto decide = [&](float zc, u64 v, u64 mask) -> u8 {
-    return zc >= threshold || (BLOOM && (v & mask) == mask) ? 1 : 0;
+  auto fetch = [&](u64 kc, float zc, Probe& p) {
+    const u64 wi = bloom_locate(blocks, hashes, klen, kc & keymask, p.mlo, p.mhi);
+    p.v = *reinterpret_cast<const uint4*>(bits + (zc >= threshold ? 0 : wi));
+  };
+  auto decide = [&](float zc, const Probe& p) -> u8 {
+    return zc >= threshold || (BLOOM && bloom_match(p.v, p.mlo, p.mhi)) ? 1 : 0;
   };
   // Full groups (all 64 lanes valid): one step decides group g (k-mers kc,
   // logits zc) while the matrix cores take group g + waves (kn), and kc's
@@ -644,12 +712,12 @@ __global__ __launch_bounds__(256) void k_prefilter(MlpDev md, float threshold, c
   // needs.
   const u64 full = count / 64;
   auto step = [&](u64& kc, float& zc, u64 kn, float& zn, u64 g) {
-    u64 v = 0, mask = 0;
-    if (BLOOM) fetch(kc, zc, v, mask);
+    Probe p{};
+    if (BLOOM) fetch(kc, zc, p);
     const float zcur = zc;
     kc = load(g + 2 * waves);
     zn = mlp64(w, oh, kn);
-    maybe[g * 64 + lane] = decide(zcur, v, mask);
+    maybe[g * 64 + lane] = decide(zcur, p);
   };
   u64 g = wave;
   u64 ka = load(g), kb = load(g + waves);
@@ -659,28 +727,28 @@ __global__ __launch_bounds__(256) void k_prefilter(MlpDev md, float threshold, c
     step(kb, zb, ka, za, g + waves);
   }
   if (g < full) {  // one full group left: its logits are in za
-    u64 v = 0, mask = 0;
-    if (BLOOM) fetch(ka, za, v, mask);
-    maybe[g * 64 + lane] = decide(za, v, mask);
+    Probe p{};
+    if (BLOOM) fetch(ka, za, p);
+    maybe[g * 64 + lane] = decide(za, p);
   }
   // the last, partial group
   if (full < groups && wave == full % waves) {
     const u64 idx = full * 64 + lane;
     const u64 kc = load(full);
     const float zc = mlp64(w, oh, kc);
-    u64 v = 0, mask = 0;
-    if (BLOOM) fetch(kc, zc, v, mask);
-    const u8 m = decide(zc, v, mask);
+    Probe p{};
+    if (BLOOM) fetch(kc, zc, p);
+    const u8 m = decide(zc, p);
     if (idx < count) maybe[idx] = m;
   }
 }
 
-// False-positive rate of a register-blocked filter (64-bit words) at
-// bits_per_key with h hashes: the load of a word is Poisson(64 / bits_per_key)
-// keys, and a non-key is accepted when all h of its bits are set.
+// False-positive rate of a register-blocked filter (128-bit blocks) at
+// bits_per_key with h hashes: the load of a block is Poisson(128 /
+// bits_per_key) keys, and a non-key is accepted when all h of its bits are set.
 static double bloom_fpr(double bits_per_key, u32 h) {
-  const double lam = 64.0 / bits_per_key;
-  const double per = std::log1p(-1.0 / 64.0) * h;  // ln P(a given bit unset by one key)
+  const double lam = 128.0 / bits_per_key;
+  const double per = std::log1p(-1.0 / 128.0) * h;  // ln P(a given bit unset by one key)
   double pmf = std::exp(-lam), s = 0.0;
   const int nmax = static_cast<int>(lam + 12.0 * std::sqrt(lam) + 24.0);
   for (int n = 0; n <= nmax; n++) {
@@ -690,13 +758,20 @@ static double bloom_fpr(double bits_per_key, u32 h) {
   return s;
 }
 
+// Keys that share a minimizer land in one block together (about 8 consecutive
+// k-mers of a sequence each), which the Poisson model does not see; genome
+// k-mers (100 Mbp-scale synthetic genomes, k = 31) need about this many more
+// bits per key for the model's rate.
+constexpr double kBloomClumpMargin = 1.2;
+
 // The smallest bits per key (on a grid of 1/20 of the unblocked filter's
-// -ln p / ln^2 2) and the hash count (1..10) that reach false-positive rate p.
+// -ln p / ln^2 2) and the hash count (1..9) at which the model reaches
+// false-positive rate p; bits_per_key includes the clump margin.
 static void bloom_size(double p, double& bits_per_key, u32& hashes) {
   const double base = -std::log(p) / (M_LN2 * M_LN2);
   bits_per_key = 0.0;
   hashes = 1;
-  for (u32 h = 1; h <= 10; h++)
+  for (u32 h = 1; h <= 9; h++)
     for (int j = 0; j <= 200; j++) {
       const double bpk = base * (1.0 + 0.05 * j);
       if (bloom_fpr(bpk, h) <= p) {
@@ -707,10 +782,11 @@ static void bloom_size(double p, double& bits_per_key, u32& hashes) {
         break;
       }
     }
-  if (bits_per_key == 0.0) {  // p beyond reach of 10 hashes in 64-bit words
+  if (bits_per_key == 0.0) {  // p beyond reach of 9 hashes in 128-bit blocks
     bits_per_key = base * 11.0;
-    hashes = 10;
+    hashes = 9;
   }
+  bits_per_key *= kBloomClumpMargin;
 }
 
 static MlpDev mlp_dev(const sbwt_mlp* nn) {
@@ -888,9 +964,11 @@ int sbwt_bloom_create(uint32_t k, uint64_t expected, double fpr, sbwt_bloom** ou
   b->keymask = k == 32 ? ~0ull : ~((1ull << (64 - 2 * k)) - 1);
   double bits_per_key = 0.0;
   bloom_size(fpr, bits_per_key, b->hashes);
-  b->blocks = expected ? static_cast<u64>(std::ceil(static_cast<double>(expected) * bits_per_key / 64.0)) : 0;
+  const double block_bits = 64.0 * kBloomBlockWords;
+  b->blocks = expected ? static_cast<u64>(std::ceil(static_cast<double>(expected) * bits_per_key / block_bits)) : 0;
+  const u64 bytes = b->blocks * kBloomBlockWords * 8;
   if (b->blocks) {
-    if (hipMalloc(&b->d_bits, b->blocks * 8) != hipSuccess || hipMemset(b->d_bits, 0, b->blocks * 8) != hipSuccess) {
+    if (hipMalloc(&b->d_bits, bytes) != hipSuccess || hipMemset(b->d_bits, 0, bytes) != hipSuccess) {
       delete b;
       return fail(SBWT_ERR_HIP, "out of device memory for a Bloom filter of %llu blocks", (unsigned long long)b->blocks);
     }
@@ -906,14 +984,14 @@ int sbwt_bloom_free(sbwt_bloom* b) {
   return SBWT_OK;
 }
 
-uint64_t sbwt_bloom_bytes(const sbwt_bloom* b) { return b ? b->blocks * 8 : 0; }
+uint64_t sbwt_bloom_bytes(const sbwt_bloom* b) { return b ? b->blocks * kBloomBlockWords * 8 : 0; }
 
 uint32_t sbwt_bloom_hashes(const sbwt_bloom* b) { return b ? b->hashes : 0; }
 
 int sbwt_bloom_get_words(const sbwt_bloom* b, uint64_t* words, uint64_t count) {
   clear_error();
-  SBWT_REQUIRE(b && words && count == b->blocks, "expected %llu words",
-               (unsigned long long)(b ? b->blocks : 0));
+  SBWT_REQUIRE(b && words && count == b->blocks * kBloomBlockWords, "expected %llu words",
+               (unsigned long long)(b ? b->blocks * kBloomBlockWords : 0));
   if (count) SBWT_HIP(hipMemcpy(words, b->d_bits, count * 8, hipMemcpyDeviceToHost));
   return SBWT_OK;
 }
@@ -923,8 +1001,8 @@ int sbwt_bloom_insert(sbwt_bloom* b, const uint64_t* d_keys, uint64_t count, voi
   SBWT_REQUIRE(b, "filter is NULL");
   if (count == 0) return SBWT_OK;
   SBWT_REQUIRE(b->blocks, "the filter was created for 0 items");
-  k_bloom_insert<<<(count + 255) / 256, 256, 0, as_stream(stream)>>>(b->d_bits, b->blocks, b->hashes, b->keymask,
-                                                                    d_keys, count);
+  k_bloom_insert<<<(count + 255) / 256, 256, 0, as_stream(stream)>>>(b->d_bits, b->blocks, b->hashes, b->k,
+                                                                    b->keymask, d_keys, count);
   SBWT_HIP(hipGetLastError());
   return SBWT_OK;
 }
@@ -933,8 +1011,22 @@ int sbwt_bloom_query(const sbwt_bloom* b, const uint64_t* d_keys, uint64_t count
   clear_error();
   SBWT_REQUIRE(b, "filter is NULL");
   if (count == 0) return SBWT_OK;
-  k_bloom_query<<<(count + 255) / 256, 256, 0, as_stream(stream)>>>(b->d_bits, b->blocks, b->hashes, b->keymask,
-                                                                   d_keys, count, d_out);
+  k_bloom_query<<<(count + 255) / 256, 256, 0, as_stream(stream)>>>(b->d_bits, b->blocks, b->hashes, b->k,
+                                                                   b->keymask, d_keys, count, d_out);
+  SBWT_HIP(hipGetLastError());
+  return SBWT_OK;
+}
+
+int sbwt_bloom_query_reads(const sbwt_bloom* b, const uint64_t* d_packed, const uint64_t* d_row_starts,
+                           const uint64_t* d_offsets, uint64_t rows, uint8_t* d_out, void* stream) {
+  clear_error();
+  SBWT_REQUIRE(b, "filter is NULL");
+  if (rows == 0) return SBWT_OK;
+  SBWT_REQUIRE(d_packed && d_row_starts && d_offsets && d_out, "device buffers must not be NULL");
+  SBWT_REQUIRE(b->blocks, "the filter was created for 0 items");
+  const u64 grid = rows < (1u << 20) ? rows : (1u << 20);
+  k_bloom_query_reads<<<grid, 64, 0, as_stream(stream)>>>(b->d_bits, b->blocks, b->hashes, b->k, b->keymask,
+                                                        d_packed, d_row_starts, d_offsets, rows, d_out);
   SBWT_HIP(hipGetLastError());
   return SBWT_OK;
 }
@@ -950,10 +1042,10 @@ int sbwt_prefilter(const sbwt_mlp* nn, float threshold, const sbwt_bloom* b, con
   const u64 groups = (count + 63) / 64;
   if (b && b->blocks)
     k_prefilter<true><<<resident_grid(k_prefilter<true>, groups), 256, 0, as_stream(stream)>>>(
-        mlp_dev(nn), threshold, b->d_bits, b->blocks, b->hashes, b->keymask, d_kmers, count, d_maybe);
+        mlp_dev(nn), threshold, b->d_bits, b->blocks, b->hashes, b->k, b->keymask, d_kmers, count, d_maybe);
   else
     k_prefilter<false><<<resident_grid(k_prefilter<false>, groups), 256, 0, as_stream(stream)>>>(
-        mlp_dev(nn), threshold, nullptr, 0, 0, ~0ull, d_kmers, count, d_maybe);
+        mlp_dev(nn), threshold, nullptr, 0, 0, 32, ~0ull, d_kmers, count, d_maybe);
   SBWT_HIP(hipGetLastError());
   return SBWT_OK;
 }

@@ -158,15 +158,40 @@ def test_bloom_matches_oracle_bit_for_bit(k, p):
     keys = rng.integers(0, 1 << 63, 300_000, dtype=np.uint64) << np.uint64(1)
     other = rng.integers(0, 1 << 63, 300_000, dtype=np.uint64) << np.uint64(1)
     bl = sg.Bloom(k, keys.size, p)
-    words, hashes = om.bloom_geometry(keys.size, p)
-    assert bl.bytes == words * 8 and bl.hashes == hashes
+    blocks, hashes = om.bloom_geometry(keys.size, p)
+    assert bl.bytes == blocks * 128 and bl.hashes == hashes
     bl.insert(dev_u64(keys))
-    want = om.bloom_build(keys, words, hashes, k)
+    want = om.bloom_build(keys, blocks, hashes, k)
     np.testing.assert_array_equal(bl.words(), want)
     assert bool((bl.query(dev_u64(keys)) == 1).all().item())
     got = bl.query(dev_u64(other)).cpu().numpy().astype(bool)
     np.testing.assert_array_equal(got, om.bloom_query(want, other, hashes, k))
     assert got.mean() <= 1.3 * p
+    bl.close()
+
+
+def test_bloom_query_reads_equals_extracted_kmers():
+    """The fused read query answers what the k-mer query answers on the
+    extracted k-mers, and what the numpy restatement answers, including rows
+    shorter than k, non-ACGT characters and a 20 kbp row."""
+    rng = np.random.default_rng(17)
+    k = 31
+    g = _genome(rng, 60_000)
+    bl = sg.Bloom(k, len(g), 0.01)
+    gk = sg.ReadBatch.from_host([g], k)
+    bl.insert(gk.kmers())
+    reads = _reads(rng, g, 200, 150) + _reads(rng, _genome(rng, 30_000), 200, 150)
+    reads += ["ACG", "", g[100:131], g[5_000:25_000], "ACGTNNACGT" * 10, g[200:400].lower()]
+    batch = sg.ReadBatch.from_host(reads, k)
+    km = batch.kmers()
+    want = bl.query(km)
+    got = bl.query_reads(batch)
+    assert torch.equal(got, want)
+    words = bl.words()
+    np.testing.assert_array_equal(got.cpu().numpy().astype(bool),
+                                  om.bloom_query(words, sg.to_u64(km), bl.hashes, k))
+    # every k-mer of the indexed sequence is in the filter
+    assert bool((bl.query_reads(gk) == 1).all().item())
     bl.close()
 
 
