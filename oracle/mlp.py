@@ -155,7 +155,7 @@ def he_params(k: int, seed: int) -> np.ndarray:
 # a key's bits lie in one 16-byte pair of 64-bit words inside a 128-byte block
 # of 16 words. block = high 64 bits of mix64(minimizer ^ BLOCK_SEED) * blocks,
 # the minimizer being the key's m-mer code c (m = min(k, 16)) with the
-# smallest o(c) = x ^ (x >> 16), x = c * 0x9E3779B1 mod 2^32 (leftmost on
+# smallest o(c) = x ^ (x >> 16), x = c * 0x9E3779 mod 2^32 (leftmost on
 # ties); h = mix64(key ^ golden), pair = h >> 61; bit i (i < hashes) = bits
 # 7i..7i+6 of mix64(h), over the pair's 128 bits (bits 0..63 in the pair's
 # first word). Sized by bloom_size: the smallest bits/key (grid of 1/20 of
@@ -206,8 +206,8 @@ def mulhi64_np(a: np.ndarray, b: int) -> np.ndarray:
 
 
 def minimizer_order(c: np.ndarray) -> np.ndarray:
-    """o(c) = x ^ (x >> 16), x = c * 0x9E3779B1 mod 2^32."""
-    x = (np.asarray(c, dtype=np.uint64) * np.uint64(0x9E3779B1)) & np.uint64(0xFFFFFFFF)
+    """o(c) = x ^ (x >> 16), x = c * 0x9E3779 mod 2^32."""
+    x = (np.asarray(c, dtype=np.uint64) * np.uint64(0x9E3779)) & np.uint64(0xFFFFFFFF)
     return x ^ (x >> np.uint64(16))
 
 

@@ -561,7 +561,7 @@ static int mlp_grad(sbwt_mlp* nn, const u64* kmers, const float* y, u64 B, hipSt
 // consecutive k-mers of a read, which mostly share it, share one cache line;
 // the pair and the bits are chosen by the key itself:
 //   minimizer = the m-mer (m = min(k, 16)) of the key whose code c has the
-//               smallest o(c) = x ^ (x >> 16), x = c * 0x9E3779B1 mod 2^32
+//               smallest o(c) = x ^ (x >> 16), x = c * 0x9E3779 mod 2^32
 //               (the leftmost one on ties)
 //   block     = fastrange(mix64(minimizer ^ kBloomBlockSeed), blocks)
 //   h         = mix64(key ^ golden); pair = h >> 61 (8 pairs per block)
@@ -579,22 +579,27 @@ __device__ __host__ __forceinline__ u64 mix64(u64 x) {
   return x;
 }
 
-// the minimizer of a top-aligned key of k characters (2m-bit code)
+// The minimizer of a top-aligned key of k characters (its 2m-bit code). With
+// m = 16 a window is one 32-bit word, so the next window's x is 4x plus the
+// new character times the multiplier (mod 2^32, the leaving character shifts
+// out): one full-rate 24-bit multiply per window. For k < 16 the one window
+// is the k-mer itself.
+constexpr u32 kMinimizerMul = 0x9E3779u;  // odd, 24 bits
+
 __device__ __forceinline__ u32 kmer_minimizer(u64 key, u32 k) {
-  const u32 m = k < 16 ? k : 16;
-  const u32 mbits = 2 * m;
-  const u32 cmask = mbits == 32 ? 0xFFFFFFFFu : ((1u << mbits) - 1u);
-  u32 best = static_cast<u32>(key >> (64 - mbits)) & cmask;
-  u32 x = best * 0x9E3779B1u;
-  u32 bo = x ^ (x >> 16);
-  for (u32 i = 1; i + m <= k; i++) {
-    const u32 c = static_cast<u32>(key >> (64 - mbits - 2 * i)) & cmask;
-    x = c * 0x9E3779B1u;
+  if (k < 16) return static_cast<u32>(key >> (64 - 2 * k));
+  const u32 lo = static_cast<u32>(key);
+  u32 x = static_cast<u32>(key >> 32) * kMinimizerMul;
+  u32 bo = x ^ (x >> 16), bi = 0;
+#pragma unroll
+  for (u32 i = 1; i <= 16; i++) {
+    if (i + 16 > k) break;  // k is the same in every lane
+    x = (x << 2) + ((lo >> (32 - 2 * i)) & 3u) * kMinimizerMul;
     const u32 o = x ^ (x >> 16);
-    best = o < bo ? c : best;
+    bi = o < bo ? i : bi;
     bo = o < bo ? o : bo;
   }
-  return best;
+  return static_cast<u32>(key >> (32 - 2 * bi));
 }
 
 // The index of the key's pair (its first word) and the pair's two masks.

@@ -134,14 +134,14 @@ def test_bloom_sizing_is_minimal_on_its_grid(p):
 
 def _minimizer_from_text(key: int, k: int) -> int:
     """The minimizer recomputed from the k-mer's characters: every m-mer as a
-    base-4 number, ordered by x ^ (x >> 16) with x = code * 0x9E3779B1 mod
+    base-4 number, ordered by x ^ (x >> 16) with x = code * 0x9E3779 mod
     2^32, the leftmost of the smallest."""
     text = "".join("ACGT"[(key >> (62 - 2 * t)) & 3] for t in range(k))
     m = min(k, 16)
     best = None
     for i in range(k - m + 1):
         code = int("".join(str("ACGT".index(ch)) for ch in text[i:i + m]), 4)
-        x = (code * 0x9E3779B1) % (1 << 32)
+        x = (code * 0x9E3779) % (1 << 32)
         o = x ^ (x >> 16)
         if best is None or o < best[0]:
             best = (o, code)
