@@ -313,7 +313,9 @@ int sbwt_bloom_query_reads(const sbwt_bloom *b, const uint64_t *d_packed,
 /* d_maybe[q] = 1 if logit(k-mer q) >= threshold or the Bloom filter (may be
  * NULL; built for the model's k) holds it; k-mers with 0 are absent from the
  * index for certain when the filter holds every indexed k-mer whose logit is
- * below threshold. Only the top 2k bits of each key are read. */
+ * below threshold. Only the top 2k bits of each key are read. Two passes on
+ * the stream: the matrix-core model, then the filter over the k-mers it
+ * rejected. */
 int sbwt_prefilter(const sbwt_mlp *nn, float threshold, const sbwt_bloom *b,
                    const uint64_t *d_kmers, uint64_t count, uint8_t *d_maybe,
                    void *stream);

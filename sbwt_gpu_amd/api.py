@@ -487,7 +487,8 @@ class Bloom:
 
 def prefilter(mlp: Mlp, threshold: float, bloom: "Bloom | None", kmers: torch.Tensor,
               out: torch.Tensor | None = None) -> torch.Tensor:
-    """maybe[q] = logit >= threshold or bloom hit (fused matrix-core kernel)."""
+    """maybe[q] = logit >= threshold (matrix cores), or else a Bloom hit (a
+    second pass over the rejected k-mers)."""
     L = _capi.load()
     if out is None:
         out = torch.empty(kmers.numel(), dtype=torch.uint8, device="cuda")

@@ -667,18 +667,13 @@ __global__ __launch_bounds__(64) void k_bloom_query_reads(const u64* __restrict_
   }
 }
 
-// maybe[q] = logit >= threshold || bloom(k-mer q); fused MLP + Bloom. Each
-// wave takes 64 k-mers per step; the Bloom blocks of the k-mers the model
-// rejects are fetched while the matrix cores run the next 64, and the k-mers
-// two steps ahead are loaded meanwhile. Every load is issued by every lane
-// in straight-line code (lanes with nothing to fetch read block 0, k-mer
-// indexes are clamped), so the wait before each use covers exactly the loads
-// it needs and never the ones still in flight.
-template <bool BLOOM>
-__global__ __launch_bounds__(256) void k_prefilter(MlpDev md, float threshold, const u64* __restrict__ bits,
-                                                   u64 blocks, u32 hashes, u32 klen, u64 keymask,
-                                                   const u64* __restrict__ kmers, u64 count,
-                                                   u8* __restrict__ maybe) {
+// maybe[q] = logit(k-mer q) >= threshold. Each wave takes 64 k-mers per step;
+// the k-mers two steps ahead are loaded while the matrix cores run. Every
+// load is issued by every lane in straight-line code (k-mer indexes are
+// clamped), so the wait before each use covers exactly the loads it needs and
+// never the ones still in flight.
+__global__ __launch_bounds__(256) void k_prefilter(MlpDev md, float threshold, const u64* __restrict__ kmers,
+                                                   u64 count, u8* __restrict__ maybe) {
   __shared__ OneHotLds oh;
   onehot_init(oh);
   MlpRegs w;
@@ -694,21 +689,8 @@ __global__ __launch_bounds__(256) void k_prefilter(MlpDev md, float threshold, c
     const u64 i = g * 64 + lane;
     return kmers[i < count ? i : count - 1];
   };
-  // Fetch the Bloom pair of k-mer kc if the model rejects it (every lane
-  // loads: lanes with nothing to fetch read pair 0).
-  struct Probe {
-    uint4 v;
-    u64 mlo, mhi;
-  };
-  auto fetch = [&](u64 kc, float zc, Probe& p) {
-    const u64 wi = bloom_locate(blocks, hashes, klen, kc & keymask, p.mlo, p.mhi);
-    p.v = *reinterpret_cast<const uint4*>(bits + (zc >= threshold ? 0 : wi));
-  };
-  auto decide = [&](float zc, const Probe& p) -> u8 {
-    return zc >= threshold || (BLOOM && bloom_match(p.v, p.mlo, p.mhi)) ? 1 : 0;
-  };
-  // Full groups (all 64 lanes valid): one step decides group g (k-mers kc,
-  // logits zc) while the matrix cores take group g + waves (kn), and kc's
+  // Full groups (all 64 lanes valid): one step stores group g's decisions
+  // (logits zc) while the matrix cores take group g + waves (kn), and kc's
   // register receives group g + 2 waves. Unrolled by two with the slots
   // swapping roles and no exit between a load and its use (so no register
   // copy of a load in flight is needed and the compiler cannot sink a load
@@ -717,12 +699,10 @@ __global__ __launch_bounds__(256) void k_prefilter(MlpDev md, float threshold, c
   // needs.
   const u64 full = count / 64;
   auto step = [&](u64& kc, float& zc, u64 kn, float& zn, u64 g) {
-    Probe p{};
-    if (BLOOM) fetch(kc, zc, p);
     const float zcur = zc;
     kc = load(g + 2 * waves);
     zn = mlp64(w, oh, kn);
-    maybe[g * 64 + lane] = decide(zcur, p);
+    maybe[g * 64 + lane] = zcur >= threshold ? 1 : 0;
   };
   u64 g = wave;
   u64 ka = load(g), kb = load(g + waves);
@@ -731,21 +711,25 @@ __global__ __launch_bounds__(256) void k_prefilter(MlpDev md, float threshold, c
     step(ka, za, kb, zb, g);
     step(kb, zb, ka, za, g + waves);
   }
-  if (g < full) {  // one full group left: its logits are in za
-    Probe p{};
-    if (BLOOM) fetch(ka, za, p);
-    maybe[g * 64 + lane] = decide(za, p);
-  }
+  if (g < full) maybe[g * 64 + lane] = za >= threshold ? 1 : 0;  // one full group left: its logits are in za
   // the last, partial group
   if (full < groups && wave == full % waves) {
     const u64 idx = full * 64 + lane;
     const u64 kc = load(full);
     const float zc = mlp64(w, oh, kc);
-    Probe p{};
-    if (BLOOM) fetch(kc, zc, p);
-    const u8 m = decide(zc, p);
-    if (idx < count) maybe[idx] = m;
+    if (idx < count) maybe[idx] = zc >= threshold ? 1 : 0;
   }
+}
+
+// maybe[q] |= bloom(k-mer q) for the k-mers the model rejected. A pass of its
+// own at full occupancy: inside the matrix-core kernel (one wave per SIMD)
+// the minimizer of every k-mer cost more than its 16-byte probe. k-mers in
+// sequence order share blocks, so their probes share lines.
+__global__ void k_bloom_or(const u64* __restrict__ bits, u64 blocks, u32 hashes, u32 k, u64 keymask,
+                           const u64* __restrict__ kmers, u64 count, u8* __restrict__ maybe) {
+  const u64 q = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x;
+  if (q >= count || maybe[q]) return;
+  maybe[q] = bloom_has(bits, blocks, hashes, k, kmers[q] & keymask) ? 1 : 0;
 }
 
 // False-positive rate of a register-blocked filter (128-bit blocks) at
@@ -1045,12 +1029,13 @@ int sbwt_prefilter(const sbwt_mlp* nn, float threshold, const sbwt_bloom* b, con
   SBWT_REQUIRE(!b || b->k == nn->shape.k, "the Bloom filter is for k = %u, the model for k = %u", b ? b->k : 0,
                nn->shape.k);
   const u64 groups = (count + 63) / 64;
-  if (b && b->blocks)
-    k_prefilter<true><<<resident_grid(k_prefilter<true>, groups), 256, 0, as_stream(stream)>>>(
-        mlp_dev(nn), threshold, b->d_bits, b->blocks, b->hashes, b->k, b->keymask, d_kmers, count, d_maybe);
-  else
-    k_prefilter<false><<<resident_grid(k_prefilter<false>, groups), 256, 0, as_stream(stream)>>>(
-        mlp_dev(nn), threshold, nullptr, 0, 0, 32, ~0ull, d_kmers, count, d_maybe);
+  k_prefilter<<<resident_grid(k_prefilter, groups), 256, 0, as_stream(stream)>>>(mlp_dev(nn), threshold, d_kmers,
+                                                                                  count, d_maybe);
+  if (b && b->blocks) {
+    SBWT_HIP(hipGetLastError());
+    k_bloom_or<<<(count + 255) / 256, 256, 0, as_stream(stream)>>>(b->d_bits, b->blocks, b->hashes, b->k,
+                                                                  b->keymask, d_kmers, count, d_maybe);
+  }
   SBWT_HIP(hipGetLastError());
   return SBWT_OK;
 }
