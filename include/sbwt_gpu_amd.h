@@ -125,7 +125,7 @@ int sbwt_index_get_rows(const sbwt_index *ix, uint64_t *const rows[4],
 int sbwt_index_write_files(const sbwt_index *ix, const char *const paths[4],
                            const char *group_path);
 
-/* (Re)builds the presearch table for m-mers, 0 <= m <= min(k, 16) (4^m x 8 bytes).
+/* (Re)builds the presearch table for m-mers, 0 <= m <= min(k, 17) (4^m x 8 bytes).
  * Replaces epic::PreSearch::call_presearch (include/presearch.hpp:32-130)
  * and the presearch kernel (include/gpu/presearch_kernel.hpp:19-37). */
 int sbwt_presearch(sbwt_index *ix, uint32_t m, void *stream);

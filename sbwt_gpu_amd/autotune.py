@@ -33,7 +33,7 @@ def autotune(index: Index, chars: torch.Tensor, row_starts: torch.Tensor, *, lay
     t_start = time.time()
     stream = torch.cuda.current_stream()
     k = index.k
-    depths = [m for m in depths if m <= min(k, 16)]
+    depths = [m for m in depths if m <= min(k, 17)]
     for layout, m in itertools.product(layouts, depths):
         if time.time() - t_start > budget_s:
             break

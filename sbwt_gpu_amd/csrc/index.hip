@@ -164,7 +164,7 @@ int build_layout(sbwt_index* ix, hipStream_t s) {
 // keeps l = r + 1 (include/gpu/presearch_kernel.hpp:19-37).
 template <int CB>
 __global__ void k_presearch(DevIndex ix, u32 m, u64 count, uint2* __restrict__ out) {
-  // grid-stride: a dispatch holds fewer than 2^32 work-items, and m = 16 has 2^32 entries
+  // grid-stride: a dispatch holds fewer than 2^32 work-items, and m >= 16 has 2^32 entries or more
   const u64 stride = gridDim.x * static_cast<u64>(blockDim.x);
   for (u64 idx = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x; idx < count; idx += stride) {
     u32 c = static_cast<u32>(idx >> (2 * (m - 1))) & 3u;

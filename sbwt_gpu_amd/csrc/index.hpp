@@ -36,7 +36,7 @@ namespace sbwt {
 
 constexpr u32 kRows = 5;    // A, C, G, T, suffix-group starts
 constexpr u32 kGroupRow = 4;
-constexpr u32 kMaxPresearch = 16;  // 4^16 entries x 8 B = 34 GB of HBM
+constexpr u32 kMaxPresearch = 17;  // 4^17 entries x 8 B = 137 GB of HBM, beside a 3 Gbp index in 288 GB
 constexpr u32 kCounters = 4;
 
 inline u64 words_for(u64 n) { return ((n + 4095) / 4096) * 64; }

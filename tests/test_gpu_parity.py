@@ -105,8 +105,9 @@ def test_search_kmers_one_mbp_scale(layout, m):
 
 
 def test_deepest_presearch_table():
-    """The largest presearch depth (16: a 32-bit table index, 4^16 entries)
-    answers like a shallow table; depth 17 is refused."""
+    """Depth 16 (4^16 entries, the first table past a 32-bit index) and the
+    largest depth, 17 (137 GB), answer like a shallow table; depth 18 is
+    refused."""
     rng = np.random.default_rng(16)
     g = genome(rng, 50_000)
     k = 20
@@ -118,8 +119,18 @@ def test_deepest_presearch_table():
     ix.presearch(16)
     assert ix.info["presearch_depth"] == 16
     np.testing.assert_array_equal(sg.to_u64(ix.search_kmers(dev_u64(q))), want)
+    batch = sg.ReadBatch.from_host([g[:5_000]], k)
+    reads_want = sg.to_u64(ix.search_reads(batch))
     with pytest.raises(sg.SbwtError, match="presearch depth"):
+        ix.presearch(18)
+    ix.presearch(0)  # frees the depth-16 table
+    torch.cuda.empty_cache()
+    if torch.cuda.mem_get_info()[0] >= (8 << 34) + (16 << 30):
         ix.presearch(17)
+        assert ix.info["presearch_depth"] == 17
+        np.testing.assert_array_equal(sg.to_u64(ix.search_kmers(dev_u64(q))), want)
+        np.testing.assert_array_equal(sg.to_u64(ix.search_reads(batch)), reads_want)
+        ix.presearch(0)
     ix.close()
 
 
