@@ -50,8 +50,8 @@ def main() -> int:
     ap.add_argument("--read-len", type=int, default=150)
     ap.add_argument("--k", type=int, default=31)
     ap.add_argument("--presearch", type=int, default=None,
-                    help="presearch depth m (default: 12, the read workloads' configured table; 16 for the "
-                         "large and pangenome indexes, where every step is an HBM line and 4^16 x 8 B = 34 GB "
+                    help="presearch depth m (default: 12, the read workloads' configured table; 17 for the "
+                         "large and pangenome indexes, where every step is an HBM line and 4^17 x 8 B = 137 GB "
                          "fits beside the index)")
     ap.add_argument("--layout", type=int, default=16)
     ap.add_argument("--seg", type=int, default=256)
@@ -94,7 +94,7 @@ def main() -> int:
     ap.add_argument("--queries", type=int, default=1_000_000_000)
     args = ap.parse_args()
     if args.presearch is None:
-        args.presearch = 16 if args.workload in ("large", "pangenome") else 12
+        args.presearch = 17 if args.workload in ("large", "pangenome") else 12
 
     if args.calib_copy:
         return calib_copy()
@@ -965,7 +965,7 @@ def cpu_baseline_kmers(ix, kmers, res_gpu, k, args):
     t = time.time()
     rows = ix.rows()
     oix = O.Index(rows[:4], ix.n, sb=1024)
-    m_cpu = min(args.presearch, 12)  # a 4^16 table is not built on the host
+    m_cpu = min(args.presearch, 12)  # a 4^16 or 4^17 table is not built on the host
     oix.presearch(m_cpu)
     log(f"oracle index + presearch built in {time.time() - t:.1f}s")
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
