@@ -341,6 +341,20 @@ int sbwt_gather_roof(const void *d_buf, uint64_t bytes, uint32_t line_bytes,
 int sbwt_parse_lines(const char *text, uint64_t len, uint8_t *chars,
                      uint64_t *row_starts, uint64_t row_cap, uint64_t *rows_out);
 
+/* FASTA / FASTQ text -> rows, one row per record. No reference counterpart:
+ * the reference reads one sequence per line, so the k-mers that span a
+ * multi-line record's line breaks are lost. The first non-empty line picks
+ * the format:
+ *   '>'  FASTA: a record's sequence lines, up to the next '>' line, joined;
+ *   '@'  FASTQ: the sequence lines up to the '+' line, then quality lines
+ *        holding as many characters as the sequence (skipped);
+ *   else one row per line, as sbwt_parse_lines.
+ * Lines end in '\n' or "\r\n"; empty lines are skipped. A record with no
+ * sequence is an empty row. A malformed FASTQ record is SBWT_ERR_FORMAT.
+ * chars needs len bytes, row_starts row_cap entries (rows + 1 are written). */
+int sbwt_parse_records(const char *text, uint64_t len, uint8_t *chars,
+                       uint64_t *row_starts, uint64_t row_cap, uint64_t *rows_out);
+
 /* Result text (ResultsPersister::save_results_to_file,
  * include/epicseq/results_persister.hpp:82-96): each answer as a signed
  * decimal (~0 -> -1) and a space, '\n' after each row; offsets[rows+1] as

@@ -83,6 +83,7 @@ SIGNATURES = {
     "sbwt_prefilter": (i32, [vp, C.c_float, vp, vp, u64, vp, vp]),
     "sbwt_gather_roof": (i32, [vp, u64, u32, u32, u32, u64, u64, vp, vp]),
     "sbwt_parse_lines": (i32, [cp, u64, vp, vp, u64, u64p]),
+    "sbwt_parse_records": (i32, [cp, u64, vp, vp, u64, u64p]),
     "sbwt_format_results": (i32, [vp, vp, u64, vp, u64, u64p]),
     "sbwt_write_results": (i32, [vp, vp, u64, i32]),
     "sbwt_reads_plan_create": (i32, [u64, u64, u32, C.POINTER(vp)]),

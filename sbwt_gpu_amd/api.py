@@ -334,6 +334,21 @@ def parse_lines(text: bytes):
     return chars[: int(rs[r])], rs[: r + 1].view(np.int64)
 
 
+def parse_records(text: bytes):
+    """FASTA / FASTQ text -> (chars uint8, row_starts int64), one row per
+    record (plain text: one row per line, as parse_lines)."""
+    L = _capi.load()
+    n = len(text)
+    chars = np.zeros(max(n, 1), np.uint8)
+    cap = text.count(b"\n") + 2
+    rs = np.zeros(cap, np.uint64)
+    rows = C.c_uint64()
+    check(L.sbwt_parse_records(text, n, C.c_void_p(chars.ctypes.data), C.c_void_p(rs.ctypes.data), cap,
+                               C.byref(rows)))
+    r = rows.value
+    return chars[: int(rs[r])], rs[: r + 1].view(np.int64)
+
+
 def format_results(results: np.ndarray, offsets: np.ndarray) -> bytes:
     """Reference result text (include/epicseq/results_persister.hpp:82-96)."""
     L = _capi.load()

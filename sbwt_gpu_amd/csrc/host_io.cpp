@@ -7,6 +7,10 @@
 // last line without '\n' keeps its last character (the reference drops it,
 // because getline's count then includes no delimiter).
 //
+// sbwt_parse_records reads FASTA and FASTQ (no reference counterpart; the
+// reference reads one sequence per line, so a multi-line record loses the
+// k-mers that span its line breaks): one row per record.
+//
 // sbwt_format_results / sbwt_write_results replace
 // ResultsPersister::save_results_to_file (include/epicseq/results_persister.hpp:82-96):
 // every answer as a signed decimal (~0 prints as -1) followed by a space, one
@@ -92,6 +96,96 @@ int sbwt_parse_lines(const char* text, uint64_t len, uint8_t* chars, uint64_t* r
       nch += le - p;
     }
     p = e + 1;
+  }
+  row_starts[rows] = nch;
+  *rows_out = rows;
+  return SBWT_OK;
+}
+
+int sbwt_parse_records(const char* text, uint64_t len, uint8_t* chars, uint64_t* row_starts, uint64_t row_cap,
+                       uint64_t* rows_out) {
+  sbwt::clear_error();
+  if ((!text && len) || !chars || !row_starts || !rows_out)
+    return sbwt::fail(SBWT_ERR_ARG, "sbwt_parse_records: NULL argument");
+  // line [b, e) starting at p ('\n' or "\r\n" ends it); returns the next line's start
+  auto line = [&](u64 p, u64& b, u64& e) -> u64 {
+    const char* nl = static_cast<const char*>(memchr(text + p, '\n', len - p));
+    const u64 end = nl ? static_cast<u64>(nl - text) : len;
+    b = p;
+    e = end > p && text[end - 1] == '\r' ? end - 1 : end;
+    return end + 1;
+  };
+  u64 p = 0, b = 0, e = 0;
+  // the first non-empty line decides the format
+  char kind = 0;
+  for (u64 q = 0; q < len;) {
+    const u64 nx = line(q, b, e);
+    if (e > b) {
+      kind = text[b];
+      break;
+    }
+    q = nx;
+  }
+  if (kind != '>' && kind != '@') return sbwt_parse_lines(text, len, chars, row_starts, row_cap, rows_out);
+  u64 rows = 0, nch = 0;
+  auto open_row = [&]() -> bool {
+    if (rows + 1 >= row_cap) return false;
+    row_starts[rows++] = nch;
+    return true;
+  };
+  auto full = [&]() {
+    return sbwt::fail(SBWT_ERR_ARG, "sbwt_parse_records: more than %llu records",
+                      static_cast<unsigned long long>(row_cap - 1));
+  };
+  if (kind == '>') {
+    bool in_record = false;
+    while (p < len) {
+      const u64 nx = line(p, b, e);
+      if (e > b && text[b] == '>') {
+        if (!open_row()) return full();
+        in_record = true;
+      } else if (e > b && in_record) {
+        memcpy(chars + nch, text + b, e - b);  // sequence lines of the record, joined
+        nch += e - b;
+      }
+      p = nx;
+    }
+  } else {
+    while (p < len) {
+      u64 nx = line(p, b, e);
+      if (e == b) {  // empty lines between records
+        p = nx;
+        continue;
+      }
+      if (text[b] != '@')
+        return sbwt::fail(SBWT_ERR_FORMAT, "sbwt_parse_records: FASTQ record %llu does not start with '@'",
+                          static_cast<unsigned long long>(rows + 1));
+      if (!open_row()) return full();
+      p = nx;
+      u64 seq = 0;
+      bool plus = false;
+      while (p < len) {  // sequence lines up to the '+' line
+        nx = line(p, b, e);
+        p = nx;
+        if (e > b && text[b] == '+') {
+          plus = true;
+          break;
+        }
+        memcpy(chars + nch, text + b, e - b);
+        nch += e - b;
+        seq += e - b;
+      }
+      u64 qual = 0;
+      while (plus && qual < seq && p < len) {  // quality lines: as many characters as the sequence
+        nx = line(p, b, e);
+        p = nx;
+        qual += e - b;
+      }
+      if (!plus || qual != seq)
+        return sbwt::fail(SBWT_ERR_FORMAT, "sbwt_parse_records: FASTQ record %llu: %s",
+                          static_cast<unsigned long long>(rows),
+                          plus ? "quality length differs from the sequence length" : "no '+' line");
+    }
   }
   row_starts[rows] = nch;
   *rows_out = rows;

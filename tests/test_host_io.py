@@ -53,3 +53,80 @@ def test_format_results_matches_oracle():
         res = rng.integers(0, 2**40, int(off[-1]), dtype=np.uint64)
         res[rng.random(res.size) < 0.3] = np.uint64(0xFFFFFFFFFFFFFFFF)
         assert sg.format_results(res, off) == O.format_results(res, off)
+
+
+# ---- FASTA / FASTQ records -------------------------------------------------
+
+_RECORD_CASES = [
+    # (text, rows)
+    (b">r1\nACGT\nTTGA\n>r2\nGG\n", [b"ACGTTTGA", b"GG"]),
+    (b">r1\r\nAC\r\nGT\r\n>r2\r\n\r\n>r3\r\nA", [b"ACGT", b"", b"A"]),
+    (b"\n\n>r1\nAC\n\nGT\n", [b"ACGT"]),
+    (b"@q1\nACGT\n+\nIIII\n@q2\nGG\n+q2\n@+\n", [b"ACGT", b"GG"]),
+    # quality lines that start with '@' and '+', sequence and quality over two lines
+    (b"@q1\nAC\nGT\n+\n@I\n+I\n@q2\nT\n+\n@\n", [b"ACGT", b"T"]),
+    (b"@q1\r\nACGT\r\n+\r\nIIII\r\n\r\n@q2\r\n\r\n+\r\n\r\n", [b"ACGT", b""]),
+    (b"ACGT\n>not a header here\nTT\n", None),  # not FASTA/FASTQ: the line parser
+    (b"", []),
+]
+
+
+@pytest.mark.parametrize("text,rows", _RECORD_CASES)
+def test_parse_records_cases(text, rows):
+    import sbwt_gpu_amd as sg
+    from oracle import records as R
+
+    c1, r1 = sg.parse_records(text)
+    c2, r2 = R.parse_records(text)
+    assert bytes(c1) == bytes(c2)
+    assert list(map(int, r1)) == list(map(int, r2))
+    if rows is not None:
+        got = [bytes(c1[int(r1[i]):int(r1[i + 1])]) for i in range(len(r1) - 1)]
+        assert got == rows
+    else:
+        c3, r3 = O.parse_lines(text)
+        assert bytes(c1) == bytes(c3) and list(map(int, r1)) == list(map(int, r3))
+
+
+@pytest.mark.parametrize("bad", [b"@q\nACGT\n", b"@q\nACGT\n+\nII\n", b"@q\nAC\n+\nII\nACGT\n"])
+def test_parse_records_rejects_malformed_fastq(bad):
+    import sbwt_gpu_amd as sg
+    from oracle import records as R
+
+    with pytest.raises(sg.SbwtError):
+        sg.parse_records(bad)
+    with pytest.raises(R.RecordError):
+        R.parse_records(bad)
+
+
+@pytest.mark.parametrize("fmt,seed", [(f, s) for f in ("fasta", "fastq") for s in range(6)])
+def test_parse_records_random(fmt, seed):
+    """Random records wrapped at random widths, LF or CRLF: rows are the
+    records' sequences, as the restatement finds them."""
+    import sbwt_gpu_amd as sg
+    from oracle import records as R
+
+    rng = np.random.default_rng(seed)
+    nl = b"\r\n" if seed % 2 else b"\n"
+    seqs, parts = [], []
+    for i in range(int(rng.integers(1, 40))):
+        s = bytes(np.frombuffer(b"ACGTNacgt", np.uint8)[rng.integers(0, 9, int(rng.integers(0, 300)))])
+        seqs.append(s)
+        w = int(rng.integers(1, 80))
+        wrap = [s[j:j + w] for j in range(0, len(s), w)] or ([b""] if fmt == "fastq" else [])
+        if fmt == "fasta":
+            parts.append(b">rec%d" % i)
+            parts.extend(wrap)
+        else:
+            qual = bytes(np.frombuffer(b"@+I#5", np.uint8)[rng.integers(0, 5, len(s))])
+            parts.append(b"@rec%d" % i)
+            parts.extend(wrap)
+            parts.append(b"+")
+            parts.extend([qual[j:j + w] for j in range(0, len(qual), w)] or [b""])
+    text = nl.join(parts) + (nl if rng.random() < 0.5 else b"")
+    c1, r1 = sg.parse_records(text)
+    c2, r2 = R.parse_records(text)
+    assert bytes(c1) == bytes(c2)
+    assert list(map(int, r1)) == list(map(int, r2))
+    got = [bytes(c1[int(r1[i]):int(r1[i + 1])]) for i in range(len(r1) - 1)]
+    assert got == seqs
