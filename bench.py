@@ -67,7 +67,8 @@ def main() -> int:
     ap.add_argument("--no-roof", action="store_true", help="skip the random-gather roof measurement")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 3 times (HBM counter calibration) and exit")
-    ap.add_argument("--workload", choices=["reads", "large", "batched", "prefilter", "pangenome"], default="reads",
+    ap.add_argument("--workload", choices=["reads", "large", "batched", "prefilter", "pangenome", "text"],
+                    default="reads",
                     help="reads: the 100 Mbp read search (default); large: a 3 Gbp uniform-random index "
                          "queried with independent 31-mers, 50%% present, through the reference's search "
                          "semantics (the HBM-bound case); batched: 10,000 uneven read sets with 5%% N, "
@@ -106,12 +107,12 @@ def main() -> int:
         # before this process touches the GPU: the counter runs are children
         traffic = measure_traffic(args)
     args.traffic = traffic
-    if args.workload in ("large", "batched", "prefilter", "pangenome"):
+    if args.workload in ("large", "batched", "prefilter", "pangenome", "text"):
         if args.gpus != 1:
             print("bench.py measures one GPU.", file=sys.stderr)
             return 2
         return {"large": run_large, "batched": run_batched, "prefilter": run_prefilter,
-                "pangenome": run_pangenome}[args.workload](args)
+                "pangenome": run_pangenome, "text": run_text}[args.workload](args)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the index is replicated per GPU and queries shard by read set "
@@ -492,6 +493,92 @@ def run_batched(args) -> int:
                    "eager_kmers_per_s": round(total / eager_s, 1), "graph_replays_equal_eager": replay_ok,
                    "autotune_best": tune["best"] if tune else None, "found_fraction": round(found / total, 4)},
         "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def run_text(args) -> int:
+    """The whole query path from query text in host memory to result text:
+    the read workload's 1M reads x 150 bp as a 151 MB text, one read per line
+    (the reference's query file format), against the 100 Mbp index; result
+    text written to /dev/null. Timed two ways on the same text: one shot
+    (parse all, copy all in, search, copy all back, print: the reference's
+    order of stages) and sbwt_search_text (batches of 16 MiB, parsing,
+    copies and printing overlapping the searches). Both write the same
+    bytes (compared on the first 20,000 lines)."""
+    import torch
+
+    import sbwt_gpu_amd as sg
+    from sbwt_gpu_amd import synth
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    k = args.k
+    g = synth.genome_codes(args.genome_bp, seed=3, gc=0.4, seg_frac=0.10, seg_len=(1_000, 50_000), seg_mut=0.01)
+    gchars = torch.from_numpy(synth.to_ascii(g)).to(dev)
+    ix = sg.Index.build(gchars, k, sg.Config(presearch_depth=args.presearch, layout=args.layout),
+                        starts=torch.tensor([0, g.size], dtype=torch.int64, device=dev))
+    del gchars
+    chars_h, _ = synth.reads(g, args.reads, args.read_len, seed=4, sub=0.01)
+    rows_h = np.asarray(chars_h, np.uint8).reshape(args.reads, args.read_len)
+    text = np.concatenate([rows_h, np.full((args.reads, 1), ord("\n"), np.uint8)], axis=1).tobytes()
+    log(f"query text: {len(text) / 1e6:.1f} MB, {args.reads} lines")
+    total = args.reads * max(args.read_len - k + 1, 0)
+    null = os.open(os.devnull, os.O_WRONLY)
+    import ctypes as C
+    import tempfile
+
+    lib = sg.load()
+
+    def one_shot(txt, fd):
+        chars, starts = sg.parse_lines(txt)
+        batch = sg.ReadBatch(torch.from_numpy(chars).to(dev), torch.from_numpy(starts.copy()).to(dev), k,
+                             host_starts=starts)
+        res = np.ascontiguousarray(ix.search_reads(batch).cpu().numpy().view(np.uint64))
+        off = np.zeros(len(starts), np.uint64)
+        off[1:] = np.cumsum(np.maximum(np.diff(starts) - k + 1, 0))
+        sg.api.check(lib.sbwt_write_results(C.c_void_p(res.ctypes.data), C.c_void_p(off.ctypes.data),
+                                            len(starts) - 1, fd))
+
+    def piped(txt, fd, batch=16 << 20):
+        return sg.search_text(ix, txt, fd=fd, batch_chars=batch)
+
+    # the same bytes either way (the first 20,000 lines, in batches of 200 kB)
+    head = text[: 20_000 * (args.read_len + 1)]
+    outs = []
+    for fn in (lambda fd: one_shot(head, fd), lambda fd: piped(head, fd, 200_000)):
+        with tempfile.TemporaryFile() as f:
+            fn(f.fileno())
+            f.seek(0)
+            outs.append(f.read())
+    same = outs[0] == outs[1] and len(outs[0]) > 0
+
+    times = {}
+    for name, fn in (("one_shot", one_shot), ("pipelined", piped)):
+        fn(text, null)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(max(args.steps // 4, 2)):
+            fn(text, null)
+        torch.cuda.synchronize()
+        times[name] = (time.perf_counter() - t0) / max(args.steps // 4, 2)
+        log(f"{name}: {total / times[name] / 1e9:.3f} G k-mers/s text to text ({times[name] * 1e3:.1f} ms)")
+    os.close(null)
+    log(f"same result text: {same}")
+    line = {
+        "metric": METRIC, "value": round(total / times["pipelined"], 1), "unit": "k-mer queries/s", "n_gpus": 1,
+        "steps": max(args.steps // 4, 2), "warmup": 1, "ms_per_step": round(times["pipelined"] * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic (seeded genome + reads generated in-run)",
+        "config": {"workload": f"query text to result text: {args.reads} reads x {args.read_len} bp as one line "
+                               f"each ({len(text) / 1e6:.0f} MB), {args.genome_bp / 1e6:g} Mbp index, k={k}, "
+                               f"presearch {args.presearch}; host memory in, result text to /dev/null",
+                   "kmers_per_step": total, "one_shot_kmers_per_s": round(total / times["one_shot"], 1),
+                   "one_shot_ms": round(times["one_shot"] * 1e3, 3),
+                   "pipelined_vs_one_shot": round(times["one_shot"] / times["pipelined"], 3),
+                   "same_result_text": same, "batch_chars": 16 << 20},
+        "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -363,6 +363,20 @@ int sbwt_format_results(const uint64_t *results, const uint64_t *offsets,
                         uint64_t rows, char *buf, uint64_t cap,
                         uint64_t *size_out);
 
+/* The same text written by the GPU: d_text[0 .. *d_size) gets the bytes
+ * sbwt_format_results gives for the count answers d_results (device) with
+ * row offsets d_offsets (device, rows + 1). If they need more than cap
+ * bytes, nothing is written and *d_size still tells how many. d_scratch
+ * holds sbwt_format_text_scratch(count) bytes. No host synchronisation.
+ * Replaces the host printing of ResultsPersister::save_results_to_file
+ * (include/epicseq/results_persister.hpp:82-96), the reference's slowest
+ * stage. */
+uint64_t sbwt_format_text_scratch(uint64_t count);
+int sbwt_format_text(const uint64_t *d_results, const uint64_t *d_offsets,
+                     uint64_t rows, uint64_t count, char *d_text, uint64_t cap,
+                     uint64_t *d_size, void *d_scratch, uint64_t scratch_bytes,
+                     void *stream);
+
 /* Same text written to a file descriptor (1 = stdout). */
 int sbwt_write_results(const uint64_t *results, const uint64_t *offsets,
                        uint64_t rows, int fd);
@@ -391,6 +405,22 @@ int sbwt_reads_plan_search(sbwt_reads_plan *plan, const sbwt_index *ix,
                            const uint64_t *d_row_starts, uint64_t rows,
                            uint64_t *d_out, uint64_t *d_offsets_out,
                            uint32_t flags, void *stream);
+
+/* The whole query path over query text in host memory, pipelined: batches
+ * of about batch_chars characters (0 = 64 MiB; rows never straddle
+ * batches), three in flight. While the device searches batch b (on a reads
+ * plan), batch b + 1 is parsed into pinned memory and copied in, and batch
+ * b - 1's result text, printed on the GPU (sbwt_format_text), is copied
+ * back and written to fd: the bytes sbwt_write_results writes (fd < 0: not
+ * written). format: 0 one row per line
+ * (sbwt_parse_lines), 1 as sbwt_parse_records (FASTA / FASTQ records, or
+ * lines). flags as sbwt_search_reads (not SBWT_COUNT). stats, when not
+ * NULL: rows, k-mers, k-mers found. Returns when every batch is written. No
+ * reference counterpart: the reference runs its stages one after another
+ * (KmerSearch::search, include/epicseq/kmer_search.hpp:97-153). */
+int sbwt_search_text(const sbwt_index *ix, const char *text, uint64_t len,
+                     uint32_t format, uint32_t flags, uint64_t batch_chars,
+                     int fd, uint64_t *stats);
 
 /* Message of the calling thread's last failure ("" if none). */
 const char *sbwt_last_error(void);

@@ -46,6 +46,9 @@ struct Parameters {
   bool store_results = true;
   bool independent = false;   // search every k-mer from the presearch table
   bool n_as_a = false;        // reference packing of non-ACGT characters
+  bool pipelined = false;     // sbwt_search_text: batches with copies, search and printing overlapped
+  bool records = false;       // FASTA / FASTQ records (one row each); implies pipelined
+  uint64_t batch_chars = 0;   // characters per pipelined batch, 0 = 64 MiB
   std::string filename_A, filename_C, filename_G, filename_T, filename_groups, fileQueries;
 
   static bool take(const char* arg, const char* key, std::string& out) {
@@ -76,6 +79,9 @@ struct Parameters {
       else if (!strcmp(a, "--do-not-store-results")) store_results = false;
       else if (!strcmp(a, "--independent")) independent = true;
       else if (!strcmp(a, "--n-as-a")) n_as_a = true;
+      else if (!strcmp(a, "--pipelined")) pipelined = true;
+      else if (!strcmp(a, "--records")) records = pipelined = true;
+      else if (take(a, "--batch-chars", v)) batch_chars = std::stoull(v);
       else {
         fprintf(stderr, "Unknown argument %s.\n", a);
         return 1;
@@ -160,6 +166,7 @@ class KmerSearch {
     std::vector<char> text(size);
     fs.seekg(0);
     fs.read(text.data(), static_cast<std::streamsize>(size));
+    if (p_.pipelined) return search_pipelined(dm, text);
     std::vector<uint8_t> chars(size);
     uint64_t cap = 2;
     for (char ch : text) cap += ch == '\n';
@@ -234,6 +241,31 @@ class KmerSearch {
   }
 
  private:
+  // Parsing, copies, search and printing in overlapping batches; the result
+  // text goes to stdout as persist() would write it (--pipelined, --records).
+  int search_pipelined(DeviceMemory& dm, const std::vector<char>& text) {
+    const uint32_t flags = (p_.n_as_a ? SBWT_READS_N_AS_A : 0u) | (p_.independent ? SBWT_READS_INDEPENDENT : 0u);
+    uint64_t stats[3] = {0, 0, 0};
+    if (!p_.store_results) fprintf(stderr, "The results are not stored to the file.\n");
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventRecord(e0, nullptr);
+    const int st = report(sbwt_search_text(dm.index, text.data(), text.size(), p_.records ? 1u : 0u, flags,
+                                           p_.batch_chars, p_.store_results ? 1 : -1, stats),
+                          "KmerSearch::search");
+    (void)hipEventRecord(e1, nullptr);
+    (void)hipEventSynchronize(e1);
+    (void)hipEventElapsedTime(&millis_search, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st) return st;
+    number_of_positions = stats[1];
+    fprintf(stderr, "Search of %llu k-mers in %llu rows, text to text: %f ms\n",
+            static_cast<unsigned long long>(stats[1]), static_cast<unsigned long long>(stats[0]), millis_search);
+    return 0;
+  }
+
   int persist() {
     ResultsPersister rp;
     return rp.save_results_to_file(results, offsets, p_);

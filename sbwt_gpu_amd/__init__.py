@@ -7,7 +7,7 @@ the library; the first call into it does, and fails loudly if it is missing.
 """
 from ._capi import SbwtError, load  # noqa: F401
 from .api import (NONE, Bloom, Config, Index, Mlp, ReadBatch, ReadsPlan, format_results, pack,  # noqa: F401
-                  parse_lines, parse_records, prefilter, to_u64)
+                  parse_lines, parse_records, prefilter, search_text, to_u64)
 
-__all__ = ["Config", "Index", "ReadBatch", "ReadsPlan", "Mlp", "Bloom", "prefilter", "pack", "parse_lines", "parse_records", "format_results", "to_u64", "NONE", "SbwtError",
+__all__ = ["Config", "Index", "ReadBatch", "ReadsPlan", "Mlp", "Bloom", "prefilter", "pack", "parse_lines", "parse_records", "search_text", "format_results", "to_u64", "NONE", "SbwtError",
            "load"]

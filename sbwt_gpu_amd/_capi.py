@@ -84,6 +84,9 @@ SIGNATURES = {
     "sbwt_gather_roof": (i32, [vp, u64, u32, u32, u32, u64, u64, vp, vp]),
     "sbwt_parse_lines": (i32, [cp, u64, vp, vp, u64, u64p]),
     "sbwt_parse_records": (i32, [cp, u64, vp, vp, u64, u64p]),
+    "sbwt_search_text": (i32, [vp, cp, u64, u32, u32, u64, i32, vp]),
+    "sbwt_format_text_scratch": (u64, [u64]),
+    "sbwt_format_text": (i32, [vp, vp, u64, u64, vp, u64, vp, vp, u64, vp]),
     "sbwt_format_results": (i32, [vp, vp, u64, vp, u64, u64p]),
     "sbwt_write_results": (i32, [vp, vp, u64, i32]),
     "sbwt_reads_plan_create": (i32, [u64, u64, u32, C.POINTER(vp)]),

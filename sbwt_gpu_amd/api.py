@@ -349,6 +349,17 @@ def parse_records(text: bytes):
     return chars[: int(rs[r])], rs[: r + 1].view(np.int64)
 
 
+def search_text(index: "Index", text: bytes, records: bool = False, fd: int = -1, batch_chars: int = 0,
+                n_as_a: bool = False, independent: bool = False) -> dict:
+    """The pipelined query path over text in host memory (sbwt_search_text):
+    result text to file descriptor fd (-1: none); returns the counts."""
+    L = _capi.load()
+    stats = (C.c_uint64 * 3)()
+    check(L.sbwt_search_text(index._h, text, len(text), 1 if records else 0,
+                             _flags(n_as_a=n_as_a, independent=independent), batch_chars, fd, stats))
+    return {"rows": stats[0], "kmers": stats[1], "found": stats[2]}
+
+
 def format_results(results: np.ndarray, offsets: np.ndarray) -> bytes:
     """Reference result text (include/epicseq/results_persister.hpp:82-96)."""
     L = _capi.load()

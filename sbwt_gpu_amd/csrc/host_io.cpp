@@ -24,12 +24,7 @@
 #include <thread>
 #include <vector>
 
-#include "../../include/sbwt_gpu_amd.h"
-
-namespace sbwt {
-int fail(int code, const char* fmt, ...);
-void clear_error();
-}  // namespace sbwt
+#include "host_io.hpp"
 
 namespace {
 
@@ -75,6 +70,153 @@ std::vector<std::string> format_all(const u64* res, const u64* off, u64 rows) {
 
 }  // namespace
 
+namespace sbwt {
+
+namespace {
+
+// Line [b, e) starting at p ('\n' or "\r\n" ends it); returns the next line's start.
+u64 line_at(const char* text, u64 len, u64 p, u64& b, u64& e) {
+  const char* nl = static_cast<const char*>(memchr(text + p, '\n', len - p));
+  const u64 end = nl ? static_cast<u64>(nl - text) : len;
+  b = p;
+  e = end > p && text[end - 1] == '\r' ? end - 1 : end;
+  return end + 1;
+}
+
+// One FASTQ record from the record start p (after empty lines): its end, or
+// ~0 if it is malformed; seq_out gets its sequence length.
+u64 fastq_record(const char* text, u64 len, u64 p, u64* seq_out, const char** why) {
+  u64 b = 0, e = 0;
+  u64 nx = line_at(text, len, p, b, e);
+  while (e == b && nx <= len) {  // empty lines between records
+    p = nx;
+    if (p >= len) return len;
+    nx = line_at(text, len, p, b, e);
+  }
+  if (text[b] != '@') {
+    *why = "does not start with '@'";
+    return ~0ull;
+  }
+  p = nx;
+  u64 seq = 0;
+  bool plus = false;
+  while (p < len) {
+    p = line_at(text, len, p, b, e);
+    if (e > b && text[b] == '+') {
+      plus = true;
+      break;
+    }
+    seq += e - b;
+  }
+  u64 qual = 0;
+  while (plus && qual < seq && p < len) {
+    p = line_at(text, len, p, b, e);
+    qual += e - b;
+  }
+  if (!plus || qual != seq) {
+    *why = plus ? "quality length differs from the sequence length" : "no '+' line";
+    return ~0ull;
+  }
+  if (seq_out) *seq_out = seq;
+  return p < len ? p : len;
+}
+
+}  // namespace
+
+TextKind text_kind(const char* text, u64 len) {
+  u64 b = 0, e = 0;
+  for (u64 q = 0; q < len;) {
+    const u64 nx = line_at(text, len, q, b, e);
+    if (e > b) return text[b] == '>' ? kTextFasta : (text[b] == '@' ? kTextFastq : kTextLines);
+    q = nx;
+  }
+  return kTextLines;
+}
+
+int parse_text(TextKind kind, const char* text, u64 len, uint8_t* chars, u64* row_starts, u64 row_cap, u64* rows_out) {
+  if (kind == kTextLines) return sbwt_parse_lines(text, len, chars, row_starts, row_cap, rows_out);
+  u64 rows = 0, nch = 0, p = 0, b = 0, e = 0;
+  auto open_row = [&]() -> bool {
+    if (rows + 1 >= row_cap) return false;
+    row_starts[rows++] = nch;
+    return true;
+  };
+  auto full = [&]() {
+    return fail(SBWT_ERR_ARG, "sbwt_parse_records: more than %llu records", static_cast<unsigned long long>(row_cap - 1));
+  };
+  if (kind == kTextFasta) {
+    bool in_record = false;
+    while (p < len) {
+      const u64 nx = line_at(text, len, p, b, e);
+      if (e > b && text[b] == '>') {
+        if (!open_row()) return full();
+        in_record = true;
+      } else if (e > b && in_record) {
+        memcpy(chars + nch, text + b, e - b);  // sequence lines of the record, joined
+        nch += e - b;
+      }
+      p = nx;
+    }
+  } else {
+    while (p < len) {
+      const char* why = "";
+      u64 seq = 0;
+      const u64 end = fastq_record(text, len, p, &seq, &why);
+      if (end == ~0ull)
+        return fail(SBWT_ERR_FORMAT, "sbwt_parse_records: FASTQ record %llu %s", static_cast<unsigned long long>(rows + 1),
+                    why);
+      // empty lines at the end of the text are no record
+      u64 q = p;
+      while (q < end && (text[q] == '\n' || text[q] == '\r')) q++;
+      if (q >= end) break;
+      if (!open_row()) return full();
+      // the sequence lines: from the header's next line up to the '+' line
+      u64 s = line_at(text, len, q, b, e);
+      while (s < end) {
+        s = line_at(text, len, s, b, e);
+        if (e > b && text[b] == '+') break;
+        memcpy(chars + nch, text + b, e - b);
+        nch += e - b;
+      }
+      p = end;
+    }
+  }
+  row_starts[rows] = nch;
+  *rows_out = rows;
+  return SBWT_OK;
+}
+
+u64 text_cut(TextKind kind, const char* text, u64 len, u64 from, u64 target, int* err) {
+  *err = SBWT_OK;
+  if (target >= len) return len;
+  u64 b = 0, e = 0;
+  if (kind == kTextFastq) {  // whole records, walked from the last cut
+    u64 p = from;
+    while (p < target && p < len) {
+      const char* why = "";
+      const u64 end = fastq_record(text, len, p, nullptr, &why);
+      if (end == ~0ull) {
+        *err = fail(SBWT_ERR_FORMAT, "sbwt_search_text: malformed FASTQ record at byte %llu: %s",
+                    static_cast<unsigned long long>(p), why);
+        return len;
+      }
+      p = end;
+    }
+    return p < len ? p : len;
+  }
+  // the first line start at or after target
+  u64 p = target;
+  if (text[p - 1] != '\n') {
+    const char* nl = static_cast<const char*>(memchr(text + p, '\n', len - p));
+    p = nl ? static_cast<u64>(nl - text) + 1 : len;
+  }
+  if (kind == kTextFasta)  // ... that starts a record
+    while (p < len && text[p] != '>') p = line_at(text, len, p, b, e);
+  return p;
+}
+
+}  // namespace sbwt
+
 extern "C" {
 
 int sbwt_parse_lines(const char* text, uint64_t len, uint8_t* chars, uint64_t* row_starts, uint64_t row_cap,
@@ -107,89 +249,7 @@ int sbwt_parse_records(const char* text, uint64_t len, uint8_t* chars, uint64_t*
   sbwt::clear_error();
   if ((!text && len) || !chars || !row_starts || !rows_out)
     return sbwt::fail(SBWT_ERR_ARG, "sbwt_parse_records: NULL argument");
-  // line [b, e) starting at p ('\n' or "\r\n" ends it); returns the next line's start
-  auto line = [&](u64 p, u64& b, u64& e) -> u64 {
-    const char* nl = static_cast<const char*>(memchr(text + p, '\n', len - p));
-    const u64 end = nl ? static_cast<u64>(nl - text) : len;
-    b = p;
-    e = end > p && text[end - 1] == '\r' ? end - 1 : end;
-    return end + 1;
-  };
-  u64 p = 0, b = 0, e = 0;
-  // the first non-empty line decides the format
-  char kind = 0;
-  for (u64 q = 0; q < len;) {
-    const u64 nx = line(q, b, e);
-    if (e > b) {
-      kind = text[b];
-      break;
-    }
-    q = nx;
-  }
-  if (kind != '>' && kind != '@') return sbwt_parse_lines(text, len, chars, row_starts, row_cap, rows_out);
-  u64 rows = 0, nch = 0;
-  auto open_row = [&]() -> bool {
-    if (rows + 1 >= row_cap) return false;
-    row_starts[rows++] = nch;
-    return true;
-  };
-  auto full = [&]() {
-    return sbwt::fail(SBWT_ERR_ARG, "sbwt_parse_records: more than %llu records",
-                      static_cast<unsigned long long>(row_cap - 1));
-  };
-  if (kind == '>') {
-    bool in_record = false;
-    while (p < len) {
-      const u64 nx = line(p, b, e);
-      if (e > b && text[b] == '>') {
-        if (!open_row()) return full();
-        in_record = true;
-      } else if (e > b && in_record) {
-        memcpy(chars + nch, text + b, e - b);  // sequence lines of the record, joined
-        nch += e - b;
-      }
-      p = nx;
-    }
-  } else {
-    while (p < len) {
-      u64 nx = line(p, b, e);
-      if (e == b) {  // empty lines between records
-        p = nx;
-        continue;
-      }
-      if (text[b] != '@')
-        return sbwt::fail(SBWT_ERR_FORMAT, "sbwt_parse_records: FASTQ record %llu does not start with '@'",
-                          static_cast<unsigned long long>(rows + 1));
-      if (!open_row()) return full();
-      p = nx;
-      u64 seq = 0;
-      bool plus = false;
-      while (p < len) {  // sequence lines up to the '+' line
-        nx = line(p, b, e);
-        p = nx;
-        if (e > b && text[b] == '+') {
-          plus = true;
-          break;
-        }
-        memcpy(chars + nch, text + b, e - b);
-        nch += e - b;
-        seq += e - b;
-      }
-      u64 qual = 0;
-      while (plus && qual < seq && p < len) {  // quality lines: as many characters as the sequence
-        nx = line(p, b, e);
-        p = nx;
-        qual += e - b;
-      }
-      if (!plus || qual != seq)
-        return sbwt::fail(SBWT_ERR_FORMAT, "sbwt_parse_records: FASTQ record %llu: %s",
-                          static_cast<unsigned long long>(rows),
-                          plus ? "quality length differs from the sequence length" : "no '+' line");
-    }
-  }
-  row_starts[rows] = nch;
-  *rows_out = rows;
-  return SBWT_OK;
+  return sbwt::parse_text(sbwt::text_kind(text, len), text, len, chars, row_starts, row_cap, rows_out);
 }
 
 int sbwt_format_results(const uint64_t* results, const uint64_t* offsets, uint64_t rows, char* buf, uint64_t cap,

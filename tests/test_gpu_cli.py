@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
+from oracle import records as R
 
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
@@ -18,7 +19,8 @@ CLI = os.path.join(ROOT, "sbwt_gpu_amd", "sbwt_search")
 ACGT = np.array(list("ACGT"))
 
 
-@pytest.mark.parametrize("extra", [[], ["--independent"], ["--without-presearch"], ["--n-as-a"], ["--layout=32"]])
+@pytest.mark.parametrize("extra", [[], ["--independent"], ["--without-presearch"], ["--n-as-a"], ["--layout=32"],
+                                   ["--pipelined", "--batch-chars=2000"], ["--pipelined", "--independent"]])
 def test_cli_matches_oracle_text(tmp_path, extra):
     rng = np.random.default_rng(8)
     g = "".join(ACGT[rng.integers(0, 4, 50_000)])
@@ -53,3 +55,37 @@ def test_cli_errors(tmp_path):
                         f"--file-g={tmp_path}/nope", f"--file-t={tmp_path}/nope", "--file-queries=x"],
                        capture_output=True, timeout=60)
     assert r.returncode == 1 and b"can not be opened" in r.stderr
+
+
+@pytest.mark.parametrize("fmt", ["fasta", "fastq"])
+def test_cli_records_match_oracle(tmp_path, fmt):
+    """--records: one row per FASTA / FASTQ record (wrapped lines, CRLF),
+    searched and printed like the oracle's search of the records."""
+    rng = np.random.default_rng(9)
+    g = "".join(ACGT[rng.integers(0, 4, 40_000)])
+    k = 25
+    s = O.SBWT([g], k)
+    paths = [str(tmp_path / f"{c}.bin") for c in "ACGT"]
+    for c in range(4):
+        O.bv_write(paths[c], s.rows[c], s.n)
+    parts = []
+    for i in range(200):
+        p = int(rng.integers(0, len(g) - 300))
+        seq = g[p:p + int(rng.integers(5, 300))]
+        wrap = [seq[j:j + 60] for j in range(0, len(seq), 60)]
+        if fmt == "fasta":
+            parts += [f">r{i}"] + wrap
+        else:
+            parts += [f"@r{i}"] + wrap + ["+"] + ["I" * len(w) for w in wrap]
+    text = ("\r\n".join(parts) + "\r\n").encode()
+    q = tmp_path / "q.txt"
+    q.write_bytes(text)
+    args = [CLI, f"--k={k}", "--k-presearch=8", *[f"--file-{c}={p}" for c, p in zip("acgt", paths)],
+            f"--file-queries={q}", "--records", "--batch-chars=3000"]
+    out = subprocess.run(args, capture_output=True, timeout=120)
+    assert out.returncode == 0, out.stderr.decode()
+    oix = O.Index(s.rows, s.n)
+    oix.presearch(8)
+    chars, rs = R.parse_records(text)
+    want = O.search_reads(oix, chars, rs, k)
+    assert out.stdout == O.format_results(want, O.kmer_offsets(rs, k))

@@ -1,0 +1,173 @@
+"""The pipelined query path (sbwt_search_text): parsing, copies, search and
+result text in overlapping batches must write exactly the text of the
+one-shot path, which is checked against the CPU oracle elsewhere."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from oracle import records as R
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():  # pragma: no cover - CPU container
+    pytest.skip("needs a GPU", allow_module_level=True)
+
+import sbwt_gpu_amd as sg  # noqa: E402
+
+ACGT = np.array(list("ACGT"))
+
+
+def _genome(rng, n):
+    return "".join(ACGT[rng.integers(0, 4, n)])
+
+
+def _reads(rng, g, count, lo=20, hi=400, n_rate=0.01):
+    out = []
+    for _ in range(count):
+        L = int(rng.integers(lo, hi))
+        p = int(rng.integers(0, len(g) - L))
+        r = np.array(list(g[p:p + L]))
+        r[rng.random(L) < 0.01] = ACGT[rng.integers(0, 4)]
+        r[rng.random(L) < n_rate] = "N"
+        out.append("".join(r))
+    return out
+
+
+@pytest.fixture(scope="module")
+def index():
+    rng = np.random.default_rng(41)
+    g = _genome(rng, 200_000)
+    ix = sg.Index.build([g], 31, sg.Config(presearch_depth=10))
+    yield ix, g
+    ix.close()
+
+
+def _expected(ix, chars, starts, k=31, **flags):
+    rows = len(starts) - 1
+    if rows == 0:
+        return b"", 0, 0
+    batch = sg.ReadBatch(torch.from_numpy(np.ascontiguousarray(chars, np.uint8).copy() if len(chars) else
+                                          np.zeros(1, np.uint8)).cuda(),
+                         torch.from_numpy(np.asarray(starts, np.int64).copy()).cuda(), k,
+                         host_starts=np.asarray(starts, np.int64))
+    res = sg.to_u64(ix.search_reads(batch, **flags))
+    lens = np.diff(np.asarray(starts, np.int64))
+    off = np.zeros(rows + 1, np.uint64)
+    off[1:] = np.cumsum(np.maximum(lens - k + 1, 0))
+    return sg.format_results(res, off), int(off[-1]), int((res != sg.NONE).sum())
+
+
+def _run(ix, text, tmp_path, **kw):
+    path = tmp_path / "out.txt"
+    fd = os.open(path, os.O_WRONLY | os.O_CREAT | os.O_TRUNC, 0o644)
+    try:
+        stats = sg.search_text(ix, text, fd=fd, **kw)
+    finally:
+        os.close(fd)
+    return path.read_bytes(), stats
+
+
+@pytest.mark.parametrize("batch", [0, 700, 5_000, 64_000])
+def test_lines_text_equals_one_shot(index, tmp_path, batch):
+    ix, g = index
+    rng = np.random.default_rng(batch + 1)
+    lines = _reads(rng, g, 1500) + ["", ">header", "ACGT", g[1000:21000]] + _reads(rng, _genome(rng, 50_000), 500)
+    rng.shuffle(lines)
+    text = ("\n".join(lines) + "\n").encode()
+    got, stats = _run(ix, text, tmp_path, batch_chars=batch)
+    chars, starts = O.parse_lines(text)
+    want, kmers, found = _expected(ix, chars, starts)
+    assert got == want
+    assert stats == {"rows": len(starts) - 1, "kmers": kmers, "found": found}
+
+
+@pytest.mark.parametrize("fmt", ["fasta", "fastq"])
+def test_records_text_equals_one_shot(index, tmp_path, fmt):
+    ix, g = index
+    rng = np.random.default_rng(7 if fmt == "fasta" else 8)
+    parts = []
+    for i, s in enumerate(_reads(rng, g, 800, 1, 600) + [g[50_000:70_000]]):
+        w = int(rng.integers(10, 90))
+        wrap = [s[j:j + w] for j in range(0, len(s), w)]
+        if fmt == "fasta":
+            parts += [f">r{i}"] + wrap
+        else:
+            q = "".join(rng.choice(list("@+I#"), len(s)))
+            parts += [f"@r{i}"] + wrap + ["+"] + [q[j:j + w] for j in range(0, len(q), w)]
+    text = ("\r\n".join(parts) + "\r\n").encode()
+    chars, starts = R.parse_records(text)
+    want, kmers, found = _expected(ix, chars, starts)
+    for batch in (3_000, 0):
+        got, stats = _run(ix, text, tmp_path, records=True, batch_chars=batch)
+        assert got == want
+        assert stats["rows"] == len(starts) - 1 and stats["kmers"] == kmers and stats["found"] == found
+
+
+def test_flags_and_edges(index, tmp_path):
+    ix, g = index
+    rng = np.random.default_rng(3)
+    text = ("\n".join(_reads(rng, g, 300, n_rate=0.05)) + "\n").encode()
+    chars, starts = O.parse_lines(text)
+    for flags in ({"n_as_a": True}, {"independent": True}):
+        got, _ = _run(ix, text, tmp_path, batch_chars=2_000, **flags)
+        assert got == _expected(ix, chars, starts, **flags)[0]
+    assert _run(ix, b"", tmp_path)[0] == b""
+    # rows shorter than k only: empty result lines, in order
+    got, stats = _run(ix, b"ACGT\nAC\n\nA\n", tmp_path, batch_chars=3)
+    assert got == b"\n\n\n" and stats == {"rows": 3, "kmers": 0, "found": 0}
+    with pytest.raises(sg.SbwtError):
+        sg.search_text(ix, b"@r\nACGT\n+\nII\n", records=True)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_format_text_on_device_equals_host(seed):
+    """The GPU's result text is byte for byte the host's (which pins the
+    reference's format against the oracle), for node ids, ~0, values past
+    2^32 and negative int64 values, rows with no answers; a buffer that is too
+    small is left untouched and the size tells what is needed."""
+    import ctypes as C
+
+    from sbwt_gpu_amd import _capi
+    from sbwt_gpu_amd.api import _ptr
+
+    L = _capi.load()
+    rng = np.random.default_rng(seed)
+    rows = int(rng.integers(0, 3000)) if seed else 0
+    counts = rng.integers(0, 40, rows)
+    counts[rng.random(rows) < 0.2] = 0
+    off = np.zeros(rows + 1, np.uint64)
+    off[1:] = np.cumsum(counts)
+    count = int(off[-1])
+    res = rng.integers(0, 200_000_000, count, dtype=np.uint64)
+    res[rng.random(count) < 0.3] = np.uint64(0xFFFFFFFFFFFFFFFF)
+    big = rng.random(count) < 0.02
+    res[big] = rng.integers(1 << 32, 1 << 63, int(big.sum()), dtype=np.uint64)
+    neg = rng.random(count) < 0.02
+    res[neg] = rng.integers(1 << 63, (1 << 64) - 1, int(neg.sum()), dtype=np.uint64)
+    if count > 3:
+        res[:3] = np.array([0, (1 << 63), (1 << 63) - 1], np.uint64)
+    want = sg.format_results(res, off)
+    d_res = torch.from_numpy(res.view(np.int64).copy() if count else np.zeros(1, np.int64)).cuda()
+    d_off = torch.from_numpy(off.view(np.int64).copy()).cuda()
+    cap = count * 21 + rows
+    d_text = torch.zeros(max(cap, 1), dtype=torch.uint8, device="cuda")
+    d_size = torch.zeros(1, dtype=torch.int64, device="cuda")
+    sb = int(L.sbwt_format_text_scratch(count))
+    scratch = torch.empty(sb, dtype=torch.uint8, device="cuda")
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _capi.check(L.sbwt_format_text(_ptr(d_res), _ptr(d_off), rows, count, _ptr(d_text), cap, _ptr(d_size),
+                                   _ptr(scratch), sb, s))
+    n = int(d_size.item())
+    assert n == len(want)
+    assert bytes(d_text[:n].cpu().numpy()) == want
+    if n > 1:
+        d_small = torch.full((n - 1,), 7, dtype=torch.uint8, device="cuda")
+        _capi.check(L.sbwt_format_text(_ptr(d_res), _ptr(d_off), rows, count, _ptr(d_small), n - 1, _ptr(d_size),
+                                       _ptr(scratch), sb, s))
+        assert int(d_size.item()) == n and bool((d_small == 7).all().item())
+    with pytest.raises(sg.SbwtError):
+        _capi.check(L.sbwt_format_text(_ptr(d_res), _ptr(d_off), rows, count, _ptr(d_text), cap, _ptr(d_size),
+                                       _ptr(scratch), sb - 1 if sb else 0, s))
