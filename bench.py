@@ -79,6 +79,7 @@ def main() -> int:
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc child runs that measure HBM traffic per launch")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--image-child", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--sets", type=int, default=10_000)
     ap.add_argument("--batch-chars", type=int, default=256 << 20)
     ap.add_argument("--no-autotune", action="store_true")
@@ -93,12 +94,16 @@ def main() -> int:
     ap.add_argument("--base-bp", type=int, default=150_000_000)
     ap.add_argument("--haplotypes", type=int, default=20)
     ap.add_argument("--queries", type=int, default=1_000_000_000)
+    ap.add_argument("--checkpoint", action="store_true",
+                    help="large: also time an index image save and load (to a temporary directory)")
     args = ap.parse_args()
     if args.presearch is None:
         args.presearch = 17 if args.workload in ("large", "pangenome") else 12
 
     if args.calib_copy:
         return calib_copy()
+    if args.image_child:
+        return image_child(args.image_child)
     traffic = None
     if args.pmc_child:
         calib_copy()
@@ -346,6 +351,7 @@ def run_large(args) -> int:
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline_kmers(ix, kmers, res, k, args)
+    checkpoint = image_round_trip(ix, kmers, out, res) if args.checkpoint else None  # closes ix
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "k-mer queries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
@@ -356,7 +362,62 @@ def run_large(args) -> int:
                    "found_fraction": round(found / Q, 4)},
         "roofline": roofline, "cpu_baseline": cpu,
     }
+    if checkpoint:
+        line["config"]["checkpoint"] = checkpoint
     print(json.dumps(line), flush=True)
+    return 0
+
+
+def image_round_trip(ix, kmers, out, res) -> dict:
+    """Checkpoint and resume: saves the index as an image, frees it, and
+    resumes it in a new process (bench.py --image-child), which loads the
+    image with the saved configuration and searches a 10M-k-mer sample of the
+    queries: wall times, file size, and whether the answers are the same."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    d = tempfile.mkdtemp(prefix="sbwt_image_")
+    try:
+        path = os.path.join(d, "index.img")
+        t = time.perf_counter()
+        ix.save(path)
+        save_s = time.perf_counter() - t
+        size = os.path.getsize(path)
+        log(f"image saved: {size / 1e9:.2f} GB in {save_s:.2f}s")
+        sample = min(kmers.numel(), 10_000_000)
+        np.save(os.path.join(d, "kmers.npy"), kmers[:sample].cpu().numpy())
+        np.save(os.path.join(d, "want.npy"), res[:sample])
+        ix.close()
+        r = subprocess.run([sys.executable, "-u", os.path.abspath(__file__), "--image-child", d],
+                           capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError(f"image child failed:\n{r.stderr[-3000:]}")
+        child = json.loads(r.stdout.strip().splitlines()[-1])
+        log(f"image resumed in a new process: loaded in {child['load_s']:.2f}s, "
+            f"sample answers equal: {child['same_answers']}")
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return {"image_bytes": size, "save_s": round(save_s, 2), **child, "sample_kmers": sample}
+
+
+def image_child(d: str) -> int:
+    import torch
+
+    import sbwt_gpu_amd as sg
+
+    torch.zeros(1, device="cuda")  # the GPU context, outside the load time
+    t = time.perf_counter()
+    ix = sg.Index.load(os.path.join(d, "index.img"))
+    torch.cuda.synchronize()
+    load_s = time.perf_counter() - t
+    kmers = torch.from_numpy(np.load(os.path.join(d, "kmers.npy"))).cuda()
+    got = ix.search_kmers(kmers).cpu().numpy().view(np.uint64)
+    same = bool(np.array_equal(got, np.load(os.path.join(d, "want.npy"))))
+    info = ix.info
+    ix.close()
+    print(json.dumps({"load_s": round(load_s, 2), "same_answers": same,
+                      "loaded_presearch_depth": info["presearch_depth"]}), flush=True)
     return 0
 
 
@@ -1001,6 +1062,7 @@ def run_pangenome(args) -> int:
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline_kmers(ix, kmers, res, k, args)
+    checkpoint = image_round_trip(ix, kmers, out, res) if args.checkpoint else None  # closes ix
     line = {
         "metric": METRIC, "value": results["plain"]["kmers_per_s"], "unit": "k-mer queries/s", "n_gpus": 1,
         "steps": nsteps, "warmup": args.warmup, "ms_per_step": results["plain"]["ms_per_step"],

@@ -114,6 +114,20 @@ int sbwt_index_build(const uint8_t *d_seq, const uint64_t *d_starts,
 /* Frees every device buffer (include/gpu/device_memory.hpp:81-121). */
 int sbwt_index_free(sbwt_index *ix);
 
+/* Checkpoint of an index: writes an image file (a 128-byte "SBWTIMG1"
+ * header with k, n, C, the configuration and a checksum, then the five rows
+ * as they sit in device memory). No reference counterpart: the reference
+ * rebuilds its index from the four bit-vector files at every start
+ * (include/matrix_sbwt.hpp:32-200). */
+int sbwt_index_save(const sbwt_index *ix, const char *path);
+
+/* Resume from an image written by sbwt_index_save: streams the rows to the
+ * device through pinned buffers, checks their checksum (SBWT_ERR_FORMAT on a
+ * damaged file) and derives the rank layout, C (checked against the header)
+ * and the presearch table on the device, with the saved configuration, or
+ * with cfg when not NULL. */
+int sbwt_index_load(const char *path, const sbwt_config *cfg, sbwt_index **out);
+
 int sbwt_index_get_info(const sbwt_index *ix, sbwt_index_info *info);
 
 /* Copies the rows back to host memory, nwords words each (>= ceil(n/64)),
@@ -406,15 +420,38 @@ int sbwt_reads_plan_search(sbwt_reads_plan *plan, const sbwt_index *ix,
                            uint64_t *d_out, uint64_t *d_offsets_out,
                            uint32_t flags, void *stream);
 
+/* The found/absent bitmap of count answers: bit (bit_offset + q) of
+ * d_bitmap is 1 iff d_results[q] != ~0 (words LSB-first; bits below
+ * bit_offset and past the last answer are 0). Writes
+ * ceil((bit_offset + count) / 64) words, so a caller can append batches
+ * at any bit. d_found, when not NULL, receives the number of found
+ * answers (a device word). No reference counterpart: the reference prints
+ * each answer as text (include/epicseq/results_persister.hpp:82-96). */
+int sbwt_found_bitmap(const uint64_t *d_results, uint64_t count,
+                      uint32_t bit_offset, uint64_t *d_bitmap,
+                      uint64_t *d_found, void *stream);
+
+/* The `format` argument of sbwt_search_text: one input kind ORed with one
+ * output kind. The binary outputs are a 16-byte header ("SBWTRES1", u32
+ * kind: 1 nodes, 2 bitmap; u32 k) and then one block per batch: u64 rows,
+ * u64 k-mers, the k-mer count of each row (rows x u64), then the answers
+ * (k-mers x u64) or their found bits (ceil(k-mers / 64) x u64, as
+ * sbwt_found_bitmap with bit_offset 0); all little-endian. */
+#define SBWT_TEXT_LINES 0u      /* one row per line (sbwt_parse_lines) */
+#define SBWT_TEXT_RECORDS 1u    /* FASTA / FASTQ records, or lines (sbwt_parse_records) */
+#define SBWT_OUTPUT_TEXT 0u     /* the result text sbwt_write_results writes */
+#define SBWT_OUTPUT_NODES 0x100u  /* binary blocks of answers */
+#define SBWT_OUTPUT_BITMAP 0x200u /* binary blocks of found bits */
+
 /* The whole query path over query text in host memory, pipelined: batches
  * of about batch_chars characters (0 = 64 MiB; rows never straddle
  * batches), three in flight. While the device searches batch b (on a reads
  * plan), batch b + 1 is parsed into pinned memory and copied in, and batch
  * b - 1's result text, printed on the GPU (sbwt_format_text), is copied
  * back and written to fd: the bytes sbwt_write_results writes (fd < 0: not
- * written). format: 0 one row per line
- * (sbwt_parse_lines), 1 as sbwt_parse_records (FASTA / FASTQ records, or
- * lines). flags as sbwt_search_reads (not SBWT_COUNT). stats, when not
+ * written). format: SBWT_TEXT_* | SBWT_OUTPUT_* (with a binary output,
+ * each batch's answers or found bits are copied back and written instead of
+ * the text). flags as sbwt_search_reads (not SBWT_COUNT). stats, when not
  * NULL: rows, k-mers, k-mers found. Returns when every batch is written. No
  * reference counterpart: the reference runs its stages one after another
  * (KmerSearch::search, include/epicseq/kmer_search.hpp:97-153). */

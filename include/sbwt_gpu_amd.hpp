@@ -49,7 +49,10 @@ struct Parameters {
   bool pipelined = false;     // sbwt_search_text: batches with copies, search and printing overlapped
   bool records = false;       // FASTA / FASTQ records (one row each); implies pipelined
   uint64_t batch_chars = 0;   // characters per pipelined batch, 0 = 64 MiB
+  uint32_t output = SBWT_OUTPUT_TEXT;  // --output=nodes|bitmap: binary results; implies pipelined
   std::string filename_A, filename_C, filename_G, filename_T, filename_groups, fileQueries;
+  std::string file_image;  // --file-image=: resume from an index image instead of the four files
+  std::string save_image;  // --save-image=: checkpoint the index to an image after creating it
 
   static bool take(const char* arg, const char* key, std::string& out) {
     const size_t n = strlen(key);
@@ -74,6 +77,8 @@ struct Parameters {
       else if (take(a, "--file-g", v)) filename_G = v;
       else if (take(a, "--file-t", v)) filename_T = v;
       else if (take(a, "--file-groups", v)) filename_groups = v;
+      else if (take(a, "--file-image", v)) file_image = v;
+      else if (take(a, "--save-image", v)) save_image = v;
       else if (take(a, "--file-queries", v)) fileQueries = v;
       else if (!strcmp(a, "--without-presearch")) with_presearch = false;
       else if (!strcmp(a, "--do-not-store-results")) store_results = false;
@@ -82,6 +87,10 @@ struct Parameters {
       else if (!strcmp(a, "--pipelined")) pipelined = true;
       else if (!strcmp(a, "--records")) records = pipelined = true;
       else if (take(a, "--batch-chars", v)) batch_chars = std::stoull(v);
+      else if (take(a, "--output", v) && (v == "text" || v == "nodes" || v == "bitmap")) {
+        output = v == "text" ? SBWT_OUTPUT_TEXT : v == "nodes" ? SBWT_OUTPUT_NODES : SBWT_OUTPUT_BITMAP;
+        if (output != SBWT_OUTPUT_TEXT) pipelined = true;
+      }
       else {
         fprintf(stderr, "Unknown argument %s.\n", a);
         return 1;
@@ -113,8 +122,10 @@ class DeviceMemory {
 
 class MatrixSBWT {
  public:
-  // Reads the four bit-vector files (and the optional group file), builds the
-  // rank layout, the C array and the presearch table in HBM.
+  // Reads the four bit-vector files (and the optional group file), or an
+  // index image (--file-image; k is the image's), builds the rank layout,
+  // the C array and the presearch table in HBM, and writes an image when
+  // asked (--save-image).
   int create(DeviceMemory& dm, Parameters& p) {
     sbwt_config cfg;
     sbwt_config_init(&cfg);
@@ -124,9 +135,18 @@ class MatrixSBWT {
     const char* paths[4] = {p.filename_A.c_str(), p.filename_C.c_str(), p.filename_G.c_str(), p.filename_T.c_str()};
     if (dm.index) sbwt_index_free(dm.index);
     dm.index = nullptr;
-    return report(sbwt_index_from_files(paths, p.filename_groups.empty() ? nullptr : p.filename_groups.c_str(),
-                                        p.k, &cfg, &dm.index),
-                  "MatrixSBWT::create");
+    if (!p.file_image.empty()) {
+      if (report(sbwt_index_load(p.file_image.c_str(), &cfg, &dm.index), "MatrixSBWT::create")) return 1;
+      sbwt_index_info info;
+      sbwt_index_get_info(dm.index, &info);
+      p.k = info.k;
+    } else if (report(sbwt_index_from_files(paths, p.filename_groups.empty() ? nullptr : p.filename_groups.c_str(),
+                                            p.k, &cfg, &dm.index),
+                      "MatrixSBWT::create")) {
+      return 1;
+    }
+    if (!p.save_image.empty()) return report(sbwt_index_save(dm.index, p.save_image.c_str()), "MatrixSBWT::create");
+    return 0;
   }
 };
 
@@ -242,7 +262,8 @@ class KmerSearch {
 
  private:
   // Parsing, copies, search and printing in overlapping batches; the result
-  // text goes to stdout as persist() would write it (--pipelined, --records).
+  // text goes to stdout as persist() would write it (--pipelined, --records),
+  // or binary results (--output=nodes|bitmap).
   int search_pipelined(DeviceMemory& dm, const std::vector<char>& text) {
     const uint32_t flags = (p_.n_as_a ? SBWT_READS_N_AS_A : 0u) | (p_.independent ? SBWT_READS_INDEPENDENT : 0u);
     uint64_t stats[3] = {0, 0, 0};
@@ -251,7 +272,8 @@ class KmerSearch {
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     (void)hipEventRecord(e0, nullptr);
-    const int st = report(sbwt_search_text(dm.index, text.data(), text.size(), p_.records ? 1u : 0u, flags,
+    const int st = report(sbwt_search_text(dm.index, text.data(), text.size(),
+                                           (p_.records ? SBWT_TEXT_RECORDS : SBWT_TEXT_LINES) | p_.output, flags,
                                            p_.batch_chars, p_.store_results ? 1 : -1, stats),
                           "KmerSearch::search");
     (void)hipEventRecord(e1, nullptr);

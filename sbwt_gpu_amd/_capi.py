@@ -17,6 +17,11 @@ COUNT = 4
 READS_PERSISTENT = 8
 READS_NO_CERT = 16
 READS_SEGMENT_LANES = 32
+TEXT_LINES = 0
+TEXT_RECORDS = 1
+OUTPUT_TEXT = 0
+OUTPUT_NODES = 0x100
+OUTPUT_BITMAP = 0x200
 
 # every function the header declares, with (restype, argtypes)
 vp, u64, u32, i32 = C.c_void_p, C.c_uint64, C.c_uint32, C.c_int
@@ -85,6 +90,9 @@ SIGNATURES = {
     "sbwt_parse_lines": (i32, [cp, u64, vp, vp, u64, u64p]),
     "sbwt_parse_records": (i32, [cp, u64, vp, vp, u64, u64p]),
     "sbwt_search_text": (i32, [vp, cp, u64, u32, u32, u64, i32, vp]),
+    "sbwt_found_bitmap": (i32, [vp, u64, u32, vp, vp, vp]),
+    "sbwt_index_save": (i32, [vp, cp]),
+    "sbwt_index_load": (i32, [cp, vp, vp]),
     "sbwt_format_text_scratch": (u64, [u64]),
     "sbwt_format_text": (i32, [vp, vp, u64, u64, vp, u64, vp, vp, u64, vp]),
     "sbwt_format_results": (i32, [vp, vp, u64, vp, u64, u64p]),

@@ -111,6 +111,23 @@ class Index:
         torch.cuda.synchronize()
         return cls(h, config)
 
+    @classmethod
+    def load(cls, path: str, config: Config | None = None) -> "Index":
+        """Resume from an index image (sbwt_index_load): the saved
+        configuration, or `config`."""
+        L = _capi.load()
+        h = C.c_void_p()
+        check(L.sbwt_index_load(path.encode(), C.byref(config.to_c()) if config else None, C.byref(h)))
+        ix = cls(h, config or Config())
+        if config is None:
+            info = ix.info
+            ix.config = Config(presearch_depth=info["presearch_depth"], layout=info["layout"])
+        return ix
+
+    def save(self, path: str) -> None:
+        """Checkpoint to an index image (sbwt_index_save)."""
+        check(self.lib.sbwt_index_save(self._h, path.encode()))
+
     def close(self) -> None:
         if self._h:
             self.lib.sbwt_index_free(self._h)
@@ -349,15 +366,71 @@ def parse_records(text: bytes):
     return chars[: int(rs[r])], rs[: r + 1].view(np.int64)
 
 
+_OUTPUTS = {"text": _capi.OUTPUT_TEXT, "nodes": _capi.OUTPUT_NODES, "bitmap": _capi.OUTPUT_BITMAP}
+
+
 def search_text(index: "Index", text: bytes, records: bool = False, fd: int = -1, batch_chars: int = 0,
-                n_as_a: bool = False, independent: bool = False) -> dict:
+                n_as_a: bool = False, independent: bool = False, output: str = "text") -> dict:
     """The pipelined query path over text in host memory (sbwt_search_text):
-    result text to file descriptor fd (-1: none); returns the counts."""
+    results to file descriptor fd (-1: none) as result text, or as binary
+    blocks of answers ("nodes") or found bits ("bitmap"; read either back
+    with read_binary_results); returns the counts."""
     L = _capi.load()
+    if output not in _OUTPUTS:
+        raise ValueError(f"output {output!r}: one of {sorted(_OUTPUTS)}")
     stats = (C.c_uint64 * 3)()
-    check(L.sbwt_search_text(index._h, text, len(text), 1 if records else 0,
+    fmt = (_capi.TEXT_RECORDS if records else _capi.TEXT_LINES) | _OUTPUTS[output]
+    check(L.sbwt_search_text(index._h, text, len(text), fmt,
                              _flags(n_as_a=n_as_a, independent=independent), batch_chars, fd, stats))
     return {"rows": stats[0], "kmers": stats[1], "found": stats[2]}
+
+
+def read_binary_results(data: bytes) -> dict:
+    """Decodes the binary output of search_text: {"kind": "nodes" | "bitmap",
+    "k", "row_kmers" (k-mers per row, uint64), "values"} where values are the
+    answers (uint64, ~0 = absent) or the found flags (bool), in row order."""
+    if len(data) < 16 or data[:8] != b"SBWTRES1":
+        raise ValueError("not a binary result file (no SBWTRES1 header)")
+    kind_code, k = np.frombuffer(data, "<u4", 2, 8)
+    if kind_code not in (1, 2):
+        raise ValueError(f"unknown result kind {kind_code}")
+    pos, counts, values = 16, [], []
+    while pos < len(data):
+        if pos + 16 > len(data):
+            raise ValueError("truncated block header")
+        rows, kmers = (int(x) for x in np.frombuffer(data, "<u8", 2, pos))
+        pos += 16
+        words = kmers if kind_code == 1 else (kmers + 63) // 64
+        if pos + 8 * (rows + words) > len(data):
+            raise ValueError("truncated block")
+        c = np.frombuffer(data, "<u8", rows, pos)
+        if int(c.sum()) != kmers:
+            raise ValueError("block row counts do not add up to its k-mers")
+        counts.append(c)
+        pos += 8 * rows
+        v = np.frombuffer(data, "<u8", words, pos)
+        pos += 8 * words
+        if kind_code == 2:
+            v = np.unpackbits(v.view(np.uint8), bitorder="little")[:kmers].astype(bool)
+        values.append(v)
+    empty = np.zeros(0, np.uint64) if kind_code == 1 else np.zeros(0, bool)
+    return {"kind": "nodes" if kind_code == 1 else "bitmap", "k": int(k),
+            "row_kmers": np.concatenate(counts) if counts else np.zeros(0, np.uint64),
+            "values": np.concatenate(values) if values else empty}
+
+
+def found_bitmap(results: torch.Tensor, bit_offset: int = 0, found: bool = False, stream=None):
+    """The found/absent bitmap of device answers (sbwt_found_bitmap): an
+    int64 tensor of ceil((bit_offset + n) / 64) words, bit bit_offset + q set
+    iff answer q is found; with found=True also the number found (a device
+    int64 scalar tensor)."""
+    L = _capi.load()
+    n = results.numel()
+    bits = torch.empty((bit_offset + n + 63) // 64, dtype=torch.int64, device=results.device)
+    cnt = torch.empty(1, dtype=torch.int64, device=results.device) if found else None
+    check(L.sbwt_found_bitmap(_ptr(results) if n else None, n, bit_offset, _ptr(bits) if bits.numel() else None,
+                              _ptr(cnt), _stream(stream)))
+    return (bits, cnt) if found else bits
 
 
 def format_results(results: np.ndarray, offsets: np.ndarray) -> bytes:

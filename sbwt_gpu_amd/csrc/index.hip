@@ -179,14 +179,21 @@ __global__ void k_presearch(DevIndex ix, u32 m, u64 count, uint2* __restrict__ o
 }
 
 static int run_presearch(sbwt_index* ix, u32 m, hipStream_t s) {
-  if (ix->d_pre) {
+  // a table of the same depth is rewritten in place: freeing and
+  // reallocating tens of GB costs the driver seconds
+  if (ix->d_pre && ix->pre_alloc_m != m) {
     SBWT_HIP(hipFree(ix->d_pre));
     ix->d_pre = nullptr;
   }
   ix->m = 0;
   if (m == 0) return SBWT_OK;
   const u64 count = 1ull << (2 * m);
-  SBWT_HIP(hipMalloc(&ix->d_pre, count * sizeof(uint2)));
+  if (!ix->d_pre) {
+    SBWT_HIP(hipMalloc(&ix->d_pre, count * sizeof(uint2)));
+    ix->pre_alloc_m = m;
+  } else {
+    SBWT_HIP(hipDeviceSynchronize());  // searches still reading the old table end first, as under hipFree
+  }
   DevIndex v = dev_view(ix);
   const u32 bt = 256;
   const u64 want = (count + bt - 1) / bt;

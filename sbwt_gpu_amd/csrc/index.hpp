@@ -19,6 +19,7 @@ struct sbwt_index {
   // presearch table of 4^m intervals
   sbwt::u32 m = 0;
   uint2* d_pre = nullptr;
+  sbwt::u32 pre_alloc_m = 0;  // depth d_pre was allocated for
   // access counters of the counted search variants (lines, steps, k-mers, full searches)
   unsigned long long* d_counters = nullptr;
   // work-item counter of the persistent read search (zeroed by each launch)

@@ -11,9 +11,11 @@
 //   compute    wait for the copy, run the batch's read search on the shared
 //              plan (one stream, so batches that share the plan's scratch
 //              run in order);
-//   compute    ... and print the batch's result text (sbwt_format_text);
-//   d2h stream wait for the search, copy the text to pinned memory;
-//   host       (at batch b + 2) wait for that copy, write the text.
+//   compute    ... and print the batch's result text (sbwt_format_text), or
+//              make its found bitmap (sbwt_found_bitmap);
+//   d2h stream wait for the search, copy the text (or the answers, or the
+//              bitmap) to pinned memory;
+//   host       (at batch b + 2) wait for that copy, write it.
 // So the host's parsing and writing and both copies overlap the searches,
 // and the printing, the reference's slowest stage, runs on the GPU.
 // Every stream keeps the default priority; pinned memory comes from
@@ -34,13 +36,15 @@ using sbwt::u64;
 struct Slot {
   uint8_t* h_chars = nullptr;
   u64* h_starts = nullptr;
-  u64* h_out = nullptr;   // the answers (only to count the found k-mers)
+  u64* h_out = nullptr;   // the answers (node output, or to count the found k-mers)
+  u64* h_bits = nullptr;  // the found bitmap (bitmap output)
   char* h_text = nullptr;
   u64* h_size = nullptr;
   uint8_t* d_chars = nullptr;
   u64* d_starts = nullptr;
   u64* d_out = nullptr;
   u64* d_offsets = nullptr;
+  u64* d_bits = nullptr;
   char* d_text = nullptr;
   u64* d_size = nullptr;
   void* d_scratch = nullptr;
@@ -60,11 +64,12 @@ struct Pipeline {
       if (s.h_chars) (void)hipHostFree(s.h_chars);
       if (s.h_starts) (void)hipHostFree(s.h_starts);
       if (s.h_out) (void)hipHostFree(s.h_out);
+      if (s.h_bits) (void)hipHostFree(s.h_bits);
       if (s.h_text) (void)hipHostFree(s.h_text);
       if (s.h_size) (void)hipHostFree(s.h_size);
       for (void* d : {static_cast<void*>(s.d_chars), static_cast<void*>(s.d_starts), static_cast<void*>(s.d_out),
-                      static_cast<void*>(s.d_offsets), static_cast<void*>(s.d_text), static_cast<void*>(s.d_size),
-                      s.d_scratch})
+                      static_cast<void*>(s.d_offsets), static_cast<void*>(s.d_bits), static_cast<void*>(s.d_text),
+                      static_cast<void*>(s.d_size), s.d_scratch})
         if (d) (void)hipFree(d);
       for (hipEvent_t e : {s.copied, s.searched, s.returned})
         if (e) (void)hipEventDestroy(e);
@@ -75,6 +80,17 @@ struct Pipeline {
   }
 };
 
+// Writes n bytes to fd (nothing when fd < 0).
+int write_all(int fd, const void* p, sbwt::u64 n) {
+  const char* c = static_cast<const char*>(p);
+  for (sbwt::u64 o = 0; fd >= 0 && o < n;) {
+    const ssize_t w = write(fd, c + o, n - o);
+    if (w < 0) return sbwt::fail(SBWT_ERR_IO, "writing the results failed");
+    o += static_cast<sbwt::u64>(w);
+  }
+  return SBWT_OK;
+}
+
 }  // namespace
 
 extern "C" int sbwt_search_text(const sbwt_index* ix, const char* text, uint64_t len, uint32_t format,
@@ -83,13 +99,28 @@ extern "C" int sbwt_search_text(const sbwt_index* ix, const char* text, uint64_t
   clear_error();
   SBWT_REQUIRE(ix, "index is NULL");
   SBWT_REQUIRE(text || len == 0, "text is NULL");
-  SBWT_REQUIRE(format <= 1, "format %u: 0 (lines) or 1 (FASTA / FASTQ records)", format);
+  const u32 input = format & 0xFFu, output = format & ~0xFFu;
+  SBWT_REQUIRE(input <= SBWT_TEXT_RECORDS, "format %#x: input kind %u is not SBWT_TEXT_LINES or SBWT_TEXT_RECORDS",
+               format, input);
+  SBWT_REQUIRE(output == SBWT_OUTPUT_TEXT || output == SBWT_OUTPUT_NODES || output == SBWT_OUTPUT_BITMAP,
+               "format %#x: output kind %#x is not SBWT_OUTPUT_TEXT, _NODES or _BITMAP", format, output);
   SBWT_REQUIRE(!(flags & SBWT_COUNT), "SBWT_COUNT is not supported here");
   if (stats) stats[0] = stats[1] = stats[2] = 0;
+  const bool text_out = output == SBWT_OUTPUT_TEXT;
+  const u32 k = ix->k;
+  if (!text_out) {
+    // binary results: "SBWTRES1", u32 kind (1 nodes, 2 bitmap), u32 k
+    unsigned char head[16] = {'S', 'B', 'W', 'T', 'R', 'E', 'S', '1'};
+    const u32 kind_code = output == SBWT_OUTPUT_NODES ? 1u : 2u;
+    for (int i = 0; i < 4; i++) {
+      head[8 + i] = static_cast<unsigned char>(kind_code >> (8 * i));
+      head[12 + i] = static_cast<unsigned char>(k >> (8 * i));
+    }
+    SBWT_TRY(write_all(fd, head, sizeof head));
+  }
   if (len == 0) return SBWT_OK;
   if (batch_chars == 0) batch_chars = 64ull << 20;
-  const TextKind kind = format ? text_kind(text, len) : kTextLines;
-  const u32 k = ix->k;
+  const TextKind kind = input ? text_kind(text, len) : kTextLines;
 
   // batch boundaries (rows never straddle a cut) and the largest batch
   std::vector<u64> cut{0};
@@ -114,46 +145,62 @@ extern "C" int sbwt_search_text(const sbwt_index* ix, const char* text, uint64_t
   u64 width = 2;
   for (u64 x = ix->n > 1 ? ix->n - 1 : 0; x >= 10; x /= 10) width++;
   if (width < 3) width = 3;
-  const u64 text_cap = chars_cap * width + rows_cap;
-  const u64 scratch = sbwt_format_text_scratch(chars_cap);
+  const u64 text_cap = text_out ? chars_cap * width + rows_cap : 0;
+  const u64 scratch = text_out ? sbwt_format_text_scratch(chars_cap) : 0;
+  const u64 bits_cap = output == SBWT_OUTPUT_BITMAP ? chars_cap / 64 + 1 : 0;
 
   Pipeline pl;
   for (hipStream_t* st : {&pl.h2d, &pl.comp, &pl.d2h}) SBWT_HIP(hipStreamCreateWithFlags(st, hipStreamNonBlocking));
   for (Slot& s : pl.slot) {
     SBWT_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_chars), chars_cap, hipHostMallocDefault));
     SBWT_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_starts), rows_cap * 8, hipHostMallocDefault));
-    if (stats) SBWT_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_out), chars_cap * 8, hipHostMallocDefault));
-    SBWT_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_text), text_cap, hipHostMallocDefault));
-    SBWT_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_size), 8, hipHostMallocDefault));
+    if ((stats && output != SBWT_OUTPUT_BITMAP) || output == SBWT_OUTPUT_NODES)
+      SBWT_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_out), chars_cap * 8, hipHostMallocDefault));
+    if (bits_cap) {
+      SBWT_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_bits), bits_cap * 8, hipHostMallocDefault));
+      SBWT_HIP(hipMalloc(&s.d_bits, bits_cap * 8));
+    }
+    if (text_out) {
+      SBWT_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_text), text_cap, hipHostMallocDefault));
+      SBWT_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_size), 8, hipHostMallocDefault));
+      SBWT_HIP(hipMalloc(&s.d_text, text_cap));
+      SBWT_HIP(hipMalloc(&s.d_size, 8));
+      SBWT_HIP(hipMalloc(&s.d_scratch, scratch));
+    }
     SBWT_HIP(hipMalloc(&s.d_chars, chars_cap));
     SBWT_HIP(hipMalloc(&s.d_starts, rows_cap * 8));
     SBWT_HIP(hipMalloc(&s.d_out, chars_cap * 8));
     SBWT_HIP(hipMalloc(&s.d_offsets, rows_cap * 8));
-    SBWT_HIP(hipMalloc(&s.d_text, text_cap));
-    SBWT_HIP(hipMalloc(&s.d_size, 8));
-    SBWT_HIP(hipMalloc(&s.d_scratch, scratch));
     for (hipEvent_t* e : {&s.copied, &s.searched, &s.returned})
       SBWT_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
   }
   SBWT_TRY(sbwt_reads_plan_create(rows_cap, chars_cap, ix->cfg.segment_kmers, &pl.plan));
 
   u64 total_rows = 0, total_kmers = 0, total_found = 0;
-  // waits for a slot's answers and writes them
+  std::vector<u64> block;
+  // waits for a slot's results and writes them
   auto finish = [&](Slot& s) -> int {
     if (!s.busy) return SBWT_OK;
     SBWT_HIP(hipEventSynchronize(s.returned));
     s.busy = false;
-    if (stats)
+    const u64 words = (s.kmers + 63) / 64;
+    if (stats && s.h_bits)
+      for (u64 w = 0; w < words; w++) total_found += __builtin_popcountll(s.h_bits[w]);
+    else if (stats)
       for (u64 q = 0; q < s.kmers; q++) total_found += s.h_out[q] != kNone;
-    const u64 size = *s.h_size;
-    if (size > s.text_bytes) return fail(SBWT_ERR_HIP, "result text of %llu bytes exceeds its %llu-byte buffer",
-                                         (unsigned long long)size, (unsigned long long)s.text_bytes);
-    for (u64 o = 0; fd >= 0 && o < size;) {
-      const ssize_t w = write(fd, s.h_text + o, size - o);
-      if (w < 0) return fail(SBWT_ERR_IO, "writing the result text failed");
-      o += static_cast<u64>(w);
+    if (text_out) {
+      const u64 size = *s.h_size;
+      if (size > s.text_bytes) return fail(SBWT_ERR_HIP, "result text of %llu bytes exceeds its %llu-byte buffer",
+                                           (unsigned long long)size, (unsigned long long)s.text_bytes);
+      return write_all(fd, s.h_text, size);
     }
-    return SBWT_OK;
+    // a block: rows, k-mers, each row's k-mer count, then the answers or bits
+    block.resize(s.rows + 2);
+    block[0] = s.rows;
+    block[1] = s.kmers;
+    for (u64 r = 0; r < s.rows; r++) block[r + 2] = s.offsets[r + 1] - s.offsets[r];
+    SBWT_TRY(write_all(fd, block.data(), block.size() * 8));
+    return s.h_bits ? write_all(fd, s.h_bits, words * 8) : write_all(fd, s.h_out, s.kmers * 8);
   };
   for (u64 b = 0; b < nb; b++) {
     Slot& s = pl.slot[b & 1];
@@ -180,14 +227,23 @@ extern "C" int sbwt_search_text(const sbwt_index* ix, const char* text, uint64_t
                                       flags, pl.comp));
     else
       SBWT_HIP(hipMemcpyAsync(s.d_offsets, s.offsets.data(), (rows + 1) * 8, hipMemcpyHostToDevice, pl.comp));
-    s.text_bytes = s.kmers * width + rows;
-    SBWT_TRY(sbwt_format_text(s.d_out, s.d_offsets, rows, s.kmers, s.d_text, s.text_bytes, s.d_size, s.d_scratch,
-                              scratch, pl.comp));
+    if (text_out) {
+      s.text_bytes = s.kmers * width + rows;
+      SBWT_TRY(sbwt_format_text(s.d_out, s.d_offsets, rows, s.kmers, s.d_text, s.text_bytes, s.d_size,
+                                s.d_scratch, scratch, pl.comp));
+    } else if (s.d_bits) {
+      SBWT_TRY(sbwt_found_bitmap(s.d_out, s.kmers, 0, s.d_bits, nullptr, pl.comp));
+    }
     SBWT_HIP(hipEventRecord(s.searched, pl.comp));
     SBWT_HIP(hipStreamWaitEvent(pl.d2h, s.searched, 0));
-    SBWT_HIP(hipMemcpyAsync(s.h_size, s.d_size, 8, hipMemcpyDeviceToHost, pl.d2h));
-    SBWT_HIP(hipMemcpyAsync(s.h_text, s.d_text, s.text_bytes, hipMemcpyDeviceToHost, pl.d2h));
-    if (stats && s.kmers) SBWT_HIP(hipMemcpyAsync(s.h_out, s.d_out, s.kmers * 8, hipMemcpyDeviceToHost, pl.d2h));
+    if (text_out) {
+      SBWT_HIP(hipMemcpyAsync(s.h_size, s.d_size, 8, hipMemcpyDeviceToHost, pl.d2h));
+      SBWT_HIP(hipMemcpyAsync(s.h_text, s.d_text, s.text_bytes, hipMemcpyDeviceToHost, pl.d2h));
+    }
+    if (s.h_bits && s.kmers)
+      SBWT_HIP(hipMemcpyAsync(s.h_bits, s.d_bits, (s.kmers + 63) / 64 * 8, hipMemcpyDeviceToHost, pl.d2h));
+    if (s.h_out && s.kmers)
+      SBWT_HIP(hipMemcpyAsync(s.h_out, s.d_out, s.kmers * 8, hipMemcpyDeviceToHost, pl.d2h));
     SBWT_HIP(hipEventRecord(s.returned, pl.d2h));
     s.busy = true;
   }

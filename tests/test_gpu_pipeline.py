@@ -171,3 +171,84 @@ def test_format_text_on_device_equals_host(seed):
     with pytest.raises(sg.SbwtError):
         _capi.check(L.sbwt_format_text(_ptr(d_res), _ptr(d_off), rows, count, _ptr(d_text), cap, _ptr(d_size),
                                        _ptr(scratch), sb - 1 if sb else 0, s))
+
+
+@pytest.mark.parametrize("output", ["nodes", "bitmap"])
+def test_binary_outputs_match_oracle(tmp_path, output):
+    """Binary results decode to the oracle's answers (nodes) or to their
+    found flags (bitmap), row by row, whatever the batch size; the blocks
+    cut rows where the text batches were cut."""
+    rng = np.random.default_rng(11)
+    g = _genome(rng, 120_000)
+    k = 31
+    s = O.SBWT([g], k)
+    oix = O.Index(s.rows, s.n)
+    oix.presearch(9)
+    ix = sg.Index.from_rows(s.rows, s.n, k, group=s.group, config=sg.Config(presearch_depth=9))
+    lines = _reads(rng, g, 600, 1, 300) + ["", "ACGT", g[:31]] + _reads(rng, _genome(rng, 9_000), 200)
+    rng.shuffle(lines)
+    text = ("\n".join(lines) + "\n").encode()
+    chars, starts = O.parse_lines(text)
+    want = O.search_reads(oix, chars, starts, k)
+    lens = np.diff(np.asarray(starts, np.int64))
+    row_kmers = np.maximum(lens - k + 1, 0).astype(np.uint64)
+    for batch in (0, 900, 7_000):
+        got, stats = _run(ix, text, tmp_path, batch_chars=batch, output=output)
+        dec = sg.read_binary_results(got)
+        assert dec["kind"] == output and dec["k"] == k
+        np.testing.assert_array_equal(dec["row_kmers"], row_kmers)
+        if output == "nodes":
+            np.testing.assert_array_equal(dec["values"], want)
+        else:
+            np.testing.assert_array_equal(dec["values"], want != sg.NONE)
+        assert stats == {"rows": len(lens), "kmers": int(row_kmers.sum()), "found": int((want != sg.NONE).sum())}
+    ix.close()
+
+
+def test_binary_output_edges(index, tmp_path):
+    ix, _ = index
+    for output, code in (("nodes", 1), ("bitmap", 2)):
+        got, stats = _run(ix, b"", tmp_path, output=output)
+        assert got == b"SBWTRES1" + np.array([code, 31], "<u4").tobytes()
+        dec = sg.read_binary_results(got)
+        assert dec["row_kmers"].size == 0 and dec["values"].size == 0
+        # rows shorter than k only (an empty line is no row, as in sbwt_parse_lines)
+        got, stats = _run(ix, b"ACGT\nAC\n\nA\n", tmp_path, output=output, batch_chars=3)
+        dec = sg.read_binary_results(got)
+        np.testing.assert_array_equal(dec["row_kmers"], [0, 0, 0])
+        assert dec["values"].size == 0 and stats == {"rows": 3, "kmers": 0, "found": 0}
+    with pytest.raises(sg.SbwtError):
+        _raw_format(ix, 0x300)
+    with pytest.raises(sg.SbwtError):
+        _raw_format(ix, 2)
+    with pytest.raises(ValueError):
+        sg.search_text(ix, b"ACGT\n", output="json")
+
+
+def _raw_format(ix, fmt):
+    from sbwt_gpu_amd import _capi
+
+    L = _capi.load()
+    _capi.check(L.sbwt_search_text(ix._h, b"ACGT\n", 5, fmt, 0, 0, -1, None))
+
+
+@pytest.mark.parametrize("count", [0, 1, 63, 64, 65, 1000, 100_003])
+def test_found_bitmap(count):
+    """sbwt_found_bitmap against numpy's bit packing, at every bit offset
+    class (0, inside the first word, the last bit of the first word), with
+    the found count."""
+    rng = np.random.default_rng(count)
+    res = rng.integers(0, 1 << 40, count, dtype=np.uint64)
+    res[rng.random(count) < 0.4] = sg.NONE
+    if count > 2:
+        res[0], res[-1] = sg.NONE, np.uint64(0)
+    d_res = torch.from_numpy(res.view(np.int64).copy()).cuda()
+    for off in (0, 1, 37, 63):
+        bits, found = sg.found_bitmap(d_res, off, found=True)
+        flags = np.concatenate([np.zeros(off, bool), res != sg.NONE])
+        nwords = (off + count + 63) // 64
+        padded = np.zeros(nwords * 64, bool)
+        padded[:flags.size] = flags
+        want = np.packbits(padded, bitorder="little").view(np.uint64)
+        np.testing.assert_array_equal(sg.to_u64(bits), want)
+        assert int(found.item()) == int((res != sg.NONE).sum())

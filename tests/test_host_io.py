@@ -130,3 +130,29 @@ def test_parse_records_random(fmt, seed):
     assert list(map(int, r1)) == list(map(int, r2))
     got = [bytes(c1[int(r1[i]):int(r1[i + 1])]) for i in range(len(r1) - 1)]
     assert got == seqs
+
+
+def test_read_binary_results_blocks():
+    """The binary result decoder: blocks concatenate in row order; bitmaps
+    are LSB-first per word and cut at each block's k-mer count; malformed
+    files raise."""
+    from sbwt_gpu_amd import read_binary_results
+
+    head = b"SBWTRES1" + np.array([1, 31], "<u4").tobytes()
+    b1 = np.array([2, 3, 1, 2, 10, 2**64 - 1, 7], "<u8").tobytes()
+    b2 = np.array([1, 0, 0], "<u8").tobytes()
+    b3 = np.array([1, 1, 1, 5], "<u8").tobytes()
+    d = read_binary_results(head + b1 + b2 + b3)
+    assert d["kind"] == "nodes" and d["k"] == 31
+    np.testing.assert_array_equal(d["row_kmers"], [1, 2, 0, 1])
+    np.testing.assert_array_equal(d["values"], np.array([10, 2**64 - 1, 7, 5], np.uint64))
+    bhead = b"SBWTRES1" + np.array([2, 25], "<u4").tobytes()
+    bits = np.array([2, 65, 64, 1], "<u8").tobytes() + np.array([1 | (1 << 63), 1], "<u8").tobytes()
+    d = read_binary_results(bhead + bits)
+    want = np.zeros(65, bool)
+    want[[0, 63, 64]] = True
+    np.testing.assert_array_equal(d["values"], want)
+    for bad in (b"SBWTRES2" + head[8:], head + b1[:-8], head + np.array([1, 4, 3], "<u8").tobytes() + b"\0" * 32,
+                b"SBWTRES1" + np.array([3, 31], "<u4").tobytes()):
+        with pytest.raises(ValueError):
+            read_binary_results(bad)
