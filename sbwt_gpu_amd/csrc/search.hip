@@ -198,7 +198,9 @@ struct Rolling {
 // the search died) triggers two short searches that can answer the next
 // absent k-mers without searching them: the k-mer starting at d (dead after
 // L characters: every k-mer containing [d, d+L-1] is absent) and the string
-// [lo-1, d] that the k-mers in between share. A dead string w proves every
+// [lo-1, d] that the k-mers in between share; when no k-mer starts at d
+// (the row's last k - 1 characters), a string [q, d] that the next k-mers up
+// to q contain, long enough to be absent by chance. A dead string w proves every
 // k-mer containing w absent, because every substring of an indexed k-mer
 // ends some node label (follow the k-mer's incoming edges back to the root).
 // With `maybe` (one byte per k-mer, in output order), k-mers marked 0 by a
@@ -294,6 +296,18 @@ __global__ __launch_bounds__(256) void k_search_reads(DevIndex ix, const u64* __
             if (search_len<CB, COUNT>(ix, extract2(packed, rs + q), static_cast<u32>(d - q + 1), t, dl) == kNone)
               clo = j + 1;
           }
+        }
+      } else if (CERT && d >= rowk && !(clo <= chi && chi > j)) {
+        // no k-mer starts at d; the k-mers j+1 .. q all contain [q, d], a
+        // string long enough to be absent by chance (log4(n) + 2 characters)
+        u64 dl = (33u - static_cast<u32>(__clz(ix.n))) / 2u + 2u;
+        if (dl > k) dl = k;
+        const u64 q = d + 1 >= dl ? (rowk - 1 < d + 1 - dl ? rowk - 1 : d + 1 - dl) : 0;
+        u32 dd;
+        if (d + 1 >= dl && q > j &&
+            search_len<CB, COUNT>(ix, extract2(packed, rs + q), static_cast<u32>(d - q + 1), t, dd) == kNone) {
+          clo = j + 1;
+          chi = q;
         }
       }
     }

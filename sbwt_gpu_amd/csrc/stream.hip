@@ -15,7 +15,9 @@
 //   sequencing error), the kernel searches the k-mer starting at d (dead
 //   after L chars, certifying the k-mers that contain [d, d+L-1]) and the
 //   string ending at d that the remaining k-mers share, so a run of up to k
-//   absent k-mers costs two short searches instead of k.
+//   absent k-mers costs two short searches instead of k. When d lies in the
+//   row's last k - 1 characters (no k-mer starts there), the next k-mers up to
+//   a start q all contain [q, d], and one search of that string answers them.
 //
 // Execution: every lane is a small state machine, and every loop iteration
 // every busy lane performs exactly one step of the same shape: two rank
@@ -175,11 +177,32 @@ struct Walker {
     if (COUNT) t.kmers += n;
   }
 
+  // A length at which a string with a sequencing error is absent from the
+  // index with high probability: log4(n) + 2 (a random string of that
+  // length occurs among n columns with probability about 1/16), at most k.
+  __device__ __forceinline__ static u32 dead_len(const DevIndex& ix) {
+    const u32 l = (33u - static_cast<u32>(__clz(ix.n))) / 2u + 2u;
+    return l < ix.k ? l : ix.k;
+  }
+
   // a k-mer before j was found absent with a dead string ending at char d
   __device__ __forceinline__ void start_cert(const DevIndex& ix, const ReadsDev& rd, UTally& t, u32 d) {
     if (!rd.cert) return;
     if (clo <= chi && chi >= j) return;  // a pending range still covers what follows
-    if (d < j || d >= rowk) return;      // nothing ahead to certify, or no k-mer starts at d
+    if (d < j) return;                   // nothing ahead to certify
+    if (d >= rowk) {
+      // No k-mer starts at d (d is in the row's last k - 1 characters). The
+      // k-mers j .. q, q <= rowk-1, all contain [q, d] (each ends after d).
+      // Far from d their searches are long (each runs to d); q is the last
+      // start whose string [q, d] is long enough to be absent from the
+      // index by chance, so one short search can answer them all.
+      const u32 q = min(rowk - 1, d + 1 - dead_len(ix));
+      if (rowk == 0 || d + 1 < dead_len(ix) || q < j) return;
+      chi = q;
+      clo = q + 1;  // empty until the search proves it (P_CERTL then sets clo = j)
+      start_search(ix, rd, t, P_CERTL, q, d - q + 1);
+      return;
+    }
     cp = d;
     start_search(ix, rd, t, P_CERTP, d, ix.k);
   }

@@ -232,6 +232,44 @@ def test_search_reads_matches_oracle(layout, seg):
             np.testing.assert_array_equal(got, want, err_msg=f"n_as_a={n_as_a} {kw}")
 
 
+@pytest.mark.parametrize("layout", [16, 32])
+def test_search_reads_errors_near_row_end(layout):
+    """Substitutions in a row's last k - 1 characters, where no k-mer starts
+    at the dead string's end: every read kernel answers as the oracle, and
+    with certificates the counted searches fall (one search of [rowk-1, d]
+    answers the row's remaining k-mers)."""
+    rng = np.random.default_rng(77 + layout)
+    g = genome(rng, 120_000)
+    k = 31
+    s = O.SBWT([g], k)
+    reads = []
+    for _ in range(600):
+        L = int(rng.integers(k, 200))
+        p = int(rng.integers(0, len(g) - L))
+        r = np.array(list(g[p:p + L]))
+        for e in rng.integers(max(0, L - k + 1), L, int(rng.integers(1, 3))):
+            r[e] = "ACGT"[("ACGT".index(r[e]) + int(rng.integers(1, 4))) % 4]
+        if rng.random() < 0.2:  # and one near the start
+            e = int(rng.integers(0, min(L, 10)))
+            r[e] = "ACGT"[("ACGT".index(r[e]) + 1) % 4]
+        reads.append("".join(r))
+    chars, rs = O.parse_lines("\n".join(reads).encode())
+    want = O.search_reads(oracle_index(s, 10), chars, rs, k)
+    ix = sg.Index.from_rows(s.rows, s.n, k, group=s.group, config=sg.Config(presearch_depth=10, layout=layout))
+    batch = sg.ReadBatch(torch.from_numpy(np.frombuffer(bytes(chars), np.uint8).copy()).cuda(), dev_u64(rs), k,
+                         seg=256, host_starts=rs)
+    searches = {}
+    for kw in READ_VARIANTS:
+        np.testing.assert_array_equal(sg.to_u64(ix.search_reads(batch, **kw)), want, err_msg=str(kw))
+        ix.counters(reset=True)
+        ix.search_reads(batch, count=True, **kw)
+        torch.cuda.synchronize()
+        searches[tuple(sorted(kw.items()))] = ix.counters(reset=True)["full_searches"]
+    assert searches[()] < searches[(("certificates", False),)]
+    assert searches[(("persistent", True),)] < searches[(("certificates", False), ("persistent", True))]
+    assert searches[(("lanes", True),)] < searches[(("certificates", False), ("lanes", True))]
+
+
 def test_search_reads_unaligned_output():
     """Answers written through an output pointer that is 8- but not
     16-byte aligned (the per-slot store path) equal the aligned ones."""
