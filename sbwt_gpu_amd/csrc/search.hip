@@ -274,7 +274,9 @@ __global__ __launch_bounds__(256) void k_search_reads(DevIndex ix, const u64* __
       } else {
         u32 dead;
         res = search_len<CB, COUNT>(ix, win, k, t, dead);
-        if (res == kNone) d = j + dead - 1;
+        // (a search that died within log4(n) + 2 characters most likely
+        // passed an error near its start; nothing to certify from its end)
+        if (res == kNone && dead >= min(k, (33u - static_cast<u32>(__clz(ix.n))) / 2u + 2u)) d = j + dead - 1;
       }
       prev = res != kNone;
       pnode = static_cast<u32>(res);

@@ -358,8 +358,12 @@ struct Walker {
     if (kfail || cdone || gscan || (strm && !found)) {
       if (kfail) {
         // a k-mer search of the k-mer at j died after i characters; a
-        // streaming or edge step failed on the k-mer's last character
-        start_cert(ix, rd, t, srch ? j - 1 + i - 1 : j - 1 + k - 1);
+        // streaming or edge step failed on the k-mer's last character. A
+        // search that died within dead_len characters most likely ran past
+        // an error near its start (the string up to the error was too short
+        // to be absent): the k-mer at its dead end then holds no error, and
+        // certifying from there would cost a whole search for nothing.
+        if (!srch || i >= dead_len(ix)) start_cert(ix, rd, t, srch ? j - 1 + i - 1 : j - 1 + k - 1);
       } else if (cdone) {
         if (purpose == P_CERTP) {
           if (salive) {
