@@ -274,19 +274,25 @@ __global__ __launch_bounds__(256) void k_search_reads(DevIndex ix, const u64* __
       } else {
         u32 dead;
         res = search_len<CB, COUNT>(ix, win, k, t, dead);
-        // (a search that died within log4(n) + 2 characters most likely
-        // passed an error near its start; nothing to certify from its end)
-        if (res == kNone && dead >= min(k, (33u - static_cast<u32>(__clz(ix.n))) / 2u + 2u)) d = j + dead - 1;
+        if (res == kNone) d = j + dead - 1;
       }
       prev = res != kNone;
       pnode = static_cast<u32>(res);
       ab.put(out, vec_out != 0, obase + j, res, j + 1 == jend);
+      // a string long enough to be absent by chance: log4(n) + 2 characters
+      const u32 dlen = min(k, (33u - static_cast<u32>(__clz(ix.n))) / 2u + 2u);
       if (CERT && d > j && d < rowk && !(clo <= chi && chi > j)) {
+        // the search from d stops alive after dlen + 2 characters (d past
+        // the error: nothing to certify, and the rest of the k-mer's search
+        // would cost a whole search for one known k-mer)
         u32 lp;
-        const u64 rp = search_len<CB, COUNT>(ix, extract2(packed, rs + d), k, t, lp);
+        const u32 cap = min(k, dlen + 2);
+        const u64 rp = search_len<CB, COUNT>(ix, extract2(packed, rs + d), cap, t, lp);
         if (rp != kNone) {
-          kpos = d;
-          knode = static_cast<u32>(rp);
+          if (cap == k) {
+            kpos = d;
+            knode = static_cast<u32>(rp);
+          }
         } else {
           const u64 lo2 = d + lp > k ? d + lp - k : 0;
           clo = lo2 > j + 1 ? lo2 : j + 1;
@@ -301,9 +307,8 @@ __global__ __launch_bounds__(256) void k_search_reads(DevIndex ix, const u64* __
         }
       } else if (CERT && d >= rowk && !(clo <= chi && chi > j)) {
         // no k-mer starts at d; the k-mers j+1 .. q all contain [q, d], a
-        // string long enough to be absent by chance (log4(n) + 2 characters)
-        u64 dl = (33u - static_cast<u32>(__clz(ix.n))) / 2u + 2u;
-        if (dl > k) dl = k;
+        // string of dlen characters
+        const u64 dl = dlen;
         const u64 q = d + 1 >= dl ? (rowk - 1 < d + 1 - dl ? rowk - 1 : d + 1 - dl) : 0;
         u32 dd;
         if (d + 1 >= dl && q > j &&

@@ -221,8 +221,14 @@ struct Walker {
       start_search(ix, rd, t, P_CERTL, q, d - q + 1);
       return;
     }
+    // The search from d certifies only if it dies, which a string starting
+    // with an error does within about dead_len characters. It stops alive
+    // after dead_len + 2: that happens when d is past the error (a search
+    // that died late because the error sat near its start), and running
+    // on to k characters would cost a whole search for one known k-mer.
     cp = d;
-    start_search(ix, rd, t, P_CERTP, d, ix.k);
+    const u32 cap = dead_len(ix) + 2;
+    start_search(ix, rd, t, P_CERTP, d, cap < ix.k ? cap : ix.k);
   }
 
   // does the k-mer at j contain a non-ACGT character?
@@ -358,17 +364,15 @@ struct Walker {
     if (kfail || cdone || gscan || (strm && !found)) {
       if (kfail) {
         // a k-mer search of the k-mer at j died after i characters; a
-        // streaming or edge step failed on the k-mer's last character. A
-        // search that died within dead_len characters most likely ran past
-        // an error near its start (the string up to the error was too short
-        // to be absent): the k-mer at its dead end then holds no error, and
-        // certifying from there would cost a whole search for nothing.
-        if (!srch || i >= dead_len(ix)) start_cert(ix, rd, t, srch ? j - 1 + i - 1 : j - 1 + k - 1);
+        // streaming or edge step failed on the k-mer's last character
+        start_cert(ix, rd, t, srch ? j - 1 + i - 1 : j - 1 + k - 1);
       } else if (cdone) {
         if (purpose == P_CERTP) {
           if (salive) {
-            kpos = cp;  // the k-mer at cp is present; answered when j reaches it
-            knode = l;
+            if (len == k) {  // the k-mer at cp is present; answered when j reaches it
+              kpos = cp;
+              knode = l;
+            }
           } else {
             const u32 lo = cp + i > k ? cp + i - k : 0u;
             clo = lo > j ? lo : j;
