@@ -38,7 +38,9 @@ namespace sbwt {
 namespace {
 
 enum : u32 { M_IDLE = 0, M_SEARCH, M_STREAM, M_EDGE, M_GSCAN };
-enum : u32 { P_KMER = 0, P_CERTP, P_CERTL };
+// P_CERTS: a P_CERTP search from the dead end of a k-mer search that died
+// early (see start_cert)
+enum : u32 { P_KMER = 0, P_CERTP, P_CERTL, P_CERTS };
 constexpr u32 kNoPos = 0xFFFFFFFFu;
 // Work items a wave claims per atomic on the shared counter (at least 64,
 // so one claim serves every lane of a refill). One atomic per lane or per
@@ -204,7 +206,9 @@ struct Walker {
   }
 
   // a k-mer before j was found absent with a dead string ending at char d
-  __device__ __forceinline__ void start_cert(const DevIndex& ix, const ReadsDev& rd, UTally& t, u32 d) {
+  // (shrt: by a search that died within dead_len characters)
+  __device__ __forceinline__ void start_cert(const DevIndex& ix, const ReadsDev& rd, UTally& t, u32 d,
+                                             bool shrt) {
     if (!rd.cert) return;
     if (clo <= chi && chi >= j) return;  // a pending range still covers what follows
     if (d < j) return;                   // nothing ahead to certify
@@ -228,7 +232,7 @@ struct Walker {
     // on to k characters would cost a whole search for one known k-mer.
     cp = d;
     const u32 cap = dead_len(ix) + 2;
-    start_search(ix, rd, t, P_CERTP, d, cap < ix.k ? cap : ix.k);
+    start_search(ix, rd, t, shrt ? P_CERTS : P_CERTP, d, cap < ix.k ? cap : ix.k);
   }
 
   // does the k-mer at j contain a non-ACGT character?
@@ -317,8 +321,12 @@ struct Walker {
   }
 
   // Apply the step: the common outcomes by selects, the rare ones in one branch.
-  __device__ __forceinline__ void update(const DevIndex& ix, const ReadsDev& rd, u64* __restrict__ out,
-                                         AnswerBuffer& ab, UTally& t, const Chunk<CB>& qa, const Chunk<CB>& qb) {
+  // skip_short: start no certificate after a search that died early (the
+  // wave's recent P_CERTS searches mostly ended alive); returns +1 / -1 when
+  // a P_CERTS search ends alive / dead, else 0.
+  __device__ __forceinline__ int update(const DevIndex& ix, const ReadsDev& rd, u64* __restrict__ out,
+                                        AnswerBuffer& ab, UTally& t, const Chunk<CB>& qa, const Chunk<CB>& qb,
+                                        bool skip_short) {
     const u32 k = ix.k;
     const Cols st = cols(ix);
     const u64 pi = st.pre ? win >> (64 - 2 * ix.m) : 0;
@@ -360,14 +368,16 @@ struct Walker {
     // rarer outcomes: certificates, group scans
     const bool kfail = kdone && !kfound;
     const bool cdone = srch && purpose != P_KMER && (sdead || salive);
+    const int cev = cdone && purpose == P_CERTS ? (salive ? 1 : -1) : 0;
     if (emit) answer(out, rd, ab, t, ev);
     if (kfail || cdone || gscan || (strm && !found)) {
       if (kfail) {
         // a k-mer search of the k-mer at j died after i characters; a
         // streaming or edge step failed on the k-mer's last character
-        start_cert(ix, rd, t, srch ? j - 1 + i - 1 : j - 1 + k - 1);
+        const bool shrt = srch && i < dead_len(ix);
+        if (!(shrt && skip_short)) start_cert(ix, rd, t, srch ? j - 1 + i - 1 : j - 1 + k - 1, shrt);
       } else if (cdone) {
-        if (purpose == P_CERTP) {
+        if (purpose == P_CERTP || purpose == P_CERTS) {
           if (salive) {
             if (len == k) {  // the k-mer at cp is present; answered when j reaches it
               kpos = cp;
@@ -396,7 +406,7 @@ struct Walker {
             mode = M_IDLE;
             prev = false;
             answer(out, rd, ab, t, kNone);  // (rare: a second call site)
-            start_cert(ix, rd, t, j - 1 + k - 1);
+            start_cert(ix, rd, t, j - 1 + k - 1, false);
           } else {
             mode = M_EDGE;
             l = g;
@@ -407,6 +417,7 @@ struct Walker {
         }
       }
     }
+    return cev;
   }
 };
 
@@ -510,6 +521,13 @@ __global__ __launch_bounds__(256) void k_reads_uniform(DevIndex ix, ReadsDev rd,
   W w[P];
   UTally t;
   ItemPool pool;
+  // The wave's tally of recent P_CERTS outcomes (alive minus dead, within
+  // +-64). A search that dies early either started at a sequencing error
+  // in the first characters of its k-mer (it then ran past the error, and
+  // the certificate search from its dead end finds a present string) or
+  // ran into a random string (the certificate search dies and certifies).
+  // Read batches mostly give the first, negative-heavy batches the second.
+  int cscore = 0;
   for (;;) {
     bool active = false;
 #pragma unroll
@@ -527,7 +545,11 @@ __global__ __launch_bounds__(256) void k_reads_uniform(DevIndex ix, ReadsDev rd,
 #pragma unroll
     for (int p = 0; p < P; p++) W::fetch(st[p], qa[p], qb[p]);
 #pragma unroll
-    for (int p = 0; p < P; p++) w[p].update(ix, rd, out, ab, t, qa[p], qb[p]);
+    for (int p = 0; p < P; p++) {
+      const int ev = w[p].update(ix, rd, out, ab, t, qa[p], qb[p], cscore > 0);
+      cscore += __popcll(__ballot(ev > 0)) - __popcll(__ballot(ev < 0));
+      cscore = cscore > 64 ? 64 : (cscore < -64 ? -64 : cscore);
+    }
   }
   if (COUNT) {
     u64 v[4] = {t.lines, t.steps, t.kmers, t.searches};

@@ -270,6 +270,39 @@ def test_search_reads_errors_near_row_end(layout):
     assert searches[(("lanes", True),)] < searches[(("certificates", False), ("lanes", True))]
 
 
+@pytest.mark.parametrize("negative", [0.1, 0.5, 0.9])
+def test_search_reads_mixed_negative_reads(negative):
+    """Batches mixing reads from the index's genome (with substitutions,
+    some near the read start) and reads from another genome, in several
+    proportions: the certificate policy of the uniform kernel adapts to the
+    mix, and every read kernel answers as the oracle."""
+    rng = np.random.default_rng(int(negative * 100))
+    g = genome(rng, 150_000)
+    other = genome(rng, 150_000)
+    k = 31
+    s = O.SBWT([g], k)
+    reads = []
+    for _ in range(1500):
+        src = other if rng.random() < negative else g
+        L = int(rng.integers(40, 250))
+        p = int(rng.integers(0, len(src) - L))
+        r = np.array(list(src[p:p + L]))
+        flip = rng.random(L) < 0.01
+        r[flip] = ACGT[rng.integers(0, 4, flip.sum())]
+        if rng.random() < 0.3:
+            e = int(rng.integers(0, 12))
+            r[e] = "ACGT"[("ACGT".index(r[e]) + 1) % 4]
+        reads.append("".join(r))
+    chars, rs = O.parse_lines("\n".join(reads).encode())
+    want = O.search_reads(oracle_index(s, 11), chars, rs, k)
+    ix = sg.Index.from_rows(s.rows, s.n, k, group=s.group, config=sg.Config(presearch_depth=11))
+    for seg in (256, 40):
+        batch = sg.ReadBatch(torch.from_numpy(np.frombuffer(bytes(chars), np.uint8).copy()).cuda(), dev_u64(rs), k,
+                             seg=seg, host_starts=rs)
+        for kw in READ_VARIANTS:
+            np.testing.assert_array_equal(sg.to_u64(ix.search_reads(batch, **kw)), want, err_msg=f"{seg} {kw}")
+
+
 def test_search_reads_unaligned_output():
     """Answers written through an output pointer that is 8- but not
     16-byte aligned (the per-slot store path) equal the aligned ones."""
