@@ -223,6 +223,8 @@ def main() -> int:
                 "traffic_detail": args.traffic,
                 "kernel": "k_reads_uniform" if args.mode in ("default", "persistent", "persistent-nocert", "persistent-independent") else "k_search_reads", "kernel_avg_ms": round(kern_avg_s * 1e3, 4),
                 "lines_per_launch": lines, "bytes_per_kmer": round(alg_bytes / total, 2),
+                "steps_per_launch": int(ctr["steps"]),
+                "floor_bytes_per_kmer": round(128 * int(ctr["steps"]) / total, 2),
                 "index_bytes": info["device_bytes"]}
     fetch = ((args.traffic or {}).get("counters") or {}).get("FETCH_SIZE")
     if fetch:
@@ -338,7 +340,9 @@ def run_large(args) -> int:
                 "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                 "traffic_detail": args.traffic, "kernel": "k_search_kmers",
                 "kernel_avg_ms": round(kern_avg_s * 1e3, 3), "lines_per_launch": lines,
-                "bytes_per_kmer": round(lines * 128 / Q, 2), "index_bytes": info["device_bytes"]}
+                "bytes_per_kmer": round(lines * 128 / Q, 2), "steps_per_launch": int(ctr["steps"]),
+                "floor_bytes_per_kmer": round(128 * int(ctr["steps"]) / Q, 2),
+                "index_bytes": info["device_bytes"]}
     fetch = ((args.traffic or {}).get("counters") or {}).get("FETCH_SIZE")
     if fetch:
         roofline["hbm_fetch_bytes_per_kmer"] = round(fetch["bytes"] / args.traffic["queries_per_launch"], 2)
