@@ -215,6 +215,97 @@ u64 text_cut(TextKind kind, const char* text, u64 len, u64 from, u64 target, int
   return p;
 }
 
+// Host threads for parsing: the machine's, at most 16.
+unsigned parse_threads() { return std::max(1u, std::min(16u, std::thread::hardware_concurrency())); }
+
+namespace {
+
+// The rows of the lines in [p, end) (sbwt_parse_lines' rules). With out
+// chars / starts NULL it only counts; otherwise row i starts at
+// base + (chars written before it).
+void lines_range(const char* text, u64 p, u64 end, uint8_t* chars, u64* starts, u64 base, u64* rows_out,
+                 u64* chars_out) {
+  u64 rows = 0, nch = 0;
+  while (p < end) {
+    const char* nl = static_cast<const char*>(memchr(text + p, '\n', end - p));
+    const u64 e = nl ? static_cast<u64>(nl - text) : end;
+    u64 le = e;
+    if (le > p && text[le - 1] == '\r') le--;
+    if (le > p && text[p] != '>') {
+      if (chars) {
+        starts[rows] = base + nch;
+        memcpy(chars + nch, text + p, le - p);
+      }
+      rows++;
+      nch += le - p;
+    }
+    p = e + 1;
+  }
+  *rows_out = rows;
+  *chars_out = nch;
+}
+
+}  // namespace
+
+int parse_lines_parallel(const char* text, u64 len, uint8_t* chars, u64* row_starts, u64 row_cap, u64* rows_out) {
+  const unsigned nt = len < (4u << 20) ? 1u : parse_threads();
+  if (row_cap == 0) return fail(SBWT_ERR_ARG, "sbwt_parse_lines: row_cap is 0");
+  if (nt == 1) {
+    // (checks the room for each row as it opens it, so a small row_cap stops early)
+    u64 rows = 0, nch = 0, p = 0;
+    while (p < len) {
+      const char* nl = static_cast<const char*>(memchr(text + p, '\n', len - p));
+      const u64 e = nl ? static_cast<u64>(nl - text) : len;
+      u64 le = e;
+      if (le > p && text[le - 1] == '\r') le--;
+      if (le > p && text[p] != '>') {
+        if (rows + 1 >= row_cap)
+          return fail(SBWT_ERR_ARG, "sbwt_parse_lines: more than %llu rows", static_cast<unsigned long long>(row_cap - 1));
+        row_starts[rows++] = nch;
+        memcpy(chars + nch, text + p, le - p);
+        nch += le - p;
+      }
+      p = e + 1;
+    }
+    row_starts[rows] = nch;
+    *rows_out = rows;
+    return SBWT_OK;
+  }
+  // pieces cut at line starts; a line never straddles two pieces
+  std::vector<u64> cut(nt + 1, len);
+  cut[0] = 0;
+  for (unsigned t = 1; t < nt; t++) {
+    u64 p = std::max(cut[t - 1], len / nt * t);
+    if (p > 0 && p < len && text[p - 1] != '\n') {
+      const char* nl = static_cast<const char*>(memchr(text + p, '\n', len - p));
+      p = nl ? static_cast<u64>(nl - text) + 1 : len;
+    }
+    cut[t] = std::min(p, len);
+  }
+  std::vector<u64> rows(nt), nch(nt);
+  auto run = [&](auto&& body) {
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nt; t++) th.emplace_back(body, t);
+    body(0u);
+    for (auto& x : th) x.join();
+  };
+  run([&](unsigned t) { lines_range(text, cut[t], cut[t + 1], nullptr, nullptr, 0, &rows[t], &nch[t]); });
+  std::vector<u64> r0(nt + 1, 0), c0(nt + 1, 0);
+  for (unsigned t = 0; t < nt; t++) {
+    r0[t + 1] = r0[t] + rows[t];
+    c0[t + 1] = c0[t] + nch[t];
+  }
+  if (r0[nt] + 1 > row_cap)
+    return fail(SBWT_ERR_ARG, "sbwt_parse_lines: more than %llu rows", static_cast<unsigned long long>(row_cap - 1));
+  run([&](unsigned t) {
+    u64 a, b;
+    lines_range(text, cut[t], cut[t + 1], chars + c0[t], row_starts + r0[t], c0[t], &a, &b);
+  });
+  row_starts[r0[nt]] = c0[nt];
+  *rows_out = r0[nt];
+  return SBWT_OK;
+}
+
 }  // namespace sbwt
 
 extern "C" {
@@ -224,24 +315,7 @@ int sbwt_parse_lines(const char* text, uint64_t len, uint8_t* chars, uint64_t* r
   sbwt::clear_error();
   if ((!text && len) || !chars || !row_starts || !rows_out)
     return sbwt::fail(SBWT_ERR_ARG, "sbwt_parse_lines: NULL argument");
-  u64 rows = 0, nch = 0, p = 0;
-  while (p < len) {
-    const char* nl = static_cast<const char*>(memchr(text + p, '\n', len - p));
-    const u64 e = nl ? static_cast<u64>(nl - text) : len;
-    u64 le = e;
-    if (le > p && text[le - 1] == '\r') le--;
-    if (le > p && text[p] != '>') {
-      if (rows + 1 >= row_cap) return sbwt::fail(SBWT_ERR_ARG, "sbwt_parse_lines: more than %llu rows",
-                                                 static_cast<unsigned long long>(row_cap - 1));
-      row_starts[rows++] = nch;
-      memcpy(chars + nch, text + p, le - p);
-      nch += le - p;
-    }
-    p = e + 1;
-  }
-  row_starts[rows] = nch;
-  *rows_out = rows;
-  return SBWT_OK;
+  return sbwt::parse_lines_parallel(text, len, chars, row_starts, row_cap, rows_out);
 }
 
 int sbwt_parse_records(const char* text, uint64_t len, uint8_t* chars, uint64_t* row_starts, uint64_t row_cap,

@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "host_io.hpp"
@@ -36,8 +37,9 @@ using sbwt::u64;
 struct Slot {
   uint8_t* h_chars = nullptr;
   u64* h_starts = nullptr;
-  u64* h_out = nullptr;   // the answers (node output, or to count the found k-mers)
-  u64* h_bits = nullptr;  // the found bitmap (bitmap output)
+  u64* h_out = nullptr;    // the answers (node output)
+  u64* h_bits = nullptr;   // the found bitmap (bitmap output)
+  u64* h_found = nullptr;  // the batch's found k-mers (stats), counted on the device
   char* h_text = nullptr;
   u64* h_size = nullptr;
   uint8_t* d_chars = nullptr;
@@ -45,6 +47,7 @@ struct Slot {
   u64* d_out = nullptr;
   u64* d_offsets = nullptr;
   u64* d_bits = nullptr;
+  u64* d_found = nullptr;
   char* d_text = nullptr;
   u64* d_size = nullptr;
   void* d_scratch = nullptr;
@@ -65,10 +68,12 @@ struct Pipeline {
       if (s.h_starts) (void)hipHostFree(s.h_starts);
       if (s.h_out) (void)hipHostFree(s.h_out);
       if (s.h_bits) (void)hipHostFree(s.h_bits);
+      if (s.h_found) (void)hipHostFree(s.h_found);
       if (s.h_text) (void)hipHostFree(s.h_text);
       if (s.h_size) (void)hipHostFree(s.h_size);
       for (void* d : {static_cast<void*>(s.d_chars), static_cast<void*>(s.d_starts), static_cast<void*>(s.d_out),
-                      static_cast<void*>(s.d_offsets), static_cast<void*>(s.d_bits), static_cast<void*>(s.d_text),
+                      static_cast<void*>(s.d_offsets), static_cast<void*>(s.d_bits), static_cast<void*>(s.d_found),
+                      static_cast<void*>(s.d_text),
                       static_cast<void*>(s.d_size), s.d_scratch})
         if (d) (void)hipFree(d);
       for (hipEvent_t e : {s.copied, s.searched, s.returned})
@@ -130,14 +135,24 @@ extern "C" int sbwt_search_text(const sbwt_index* ix, const char* text, uint64_t
     u64 next = text_cut(kind, text, len, cut.back(), cut.back() + batch_chars, &err);
     if (err != SBWT_OK) return err;
     if (next <= cut.back()) next = len;
-    u64 lines = 0;
-    for (const char* q = text + cut.back(); (q = static_cast<const char*>(memchr(q, '\n', text + next - q)));) {
-      lines++;
-      q++;
-    }
     max_bytes = std::max(max_bytes, next - cut.back());
-    max_lines = std::max(max_lines, lines);
     cut.push_back(next);
+  }
+  {
+    // newlines per batch (an upper bound of its rows), counted on host threads
+    std::vector<u64> lines(cut.size() - 1, 0);
+    const unsigned nt = std::min<u64>(parse_threads(), lines.size());
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; t++)
+      th.emplace_back([&, t] {
+        for (u64 b = t; b < lines.size(); b += nt)
+          for (const char* q = text + cut[b]; (q = static_cast<const char*>(memchr(q, '\n', text + cut[b + 1] - q)));) {
+            lines[b]++;
+            q++;
+          }
+      });
+    for (auto& x : th) x.join();
+    for (u64 l : lines) max_lines = std::max(max_lines, l);
   }
   const u64 rows_cap = max_lines + 2, chars_cap = max_bytes;
   const u64 nb = cut.size() - 1;
@@ -147,18 +162,22 @@ extern "C" int sbwt_search_text(const sbwt_index* ix, const char* text, uint64_t
   if (width < 3) width = 3;
   const u64 text_cap = text_out ? chars_cap * width + rows_cap : 0;
   const u64 scratch = text_out ? sbwt_format_text_scratch(chars_cap) : 0;
-  const u64 bits_cap = output == SBWT_OUTPUT_BITMAP ? chars_cap / 64 + 1 : 0;
+  // the bitmap is also how the found k-mers are counted (stats)
+  const u64 bits_cap = output == SBWT_OUTPUT_BITMAP || stats ? chars_cap / 64 + 1 : 0;
 
   Pipeline pl;
   for (hipStream_t* st : {&pl.h2d, &pl.comp, &pl.d2h}) SBWT_HIP(hipStreamCreateWithFlags(st, hipStreamNonBlocking));
   for (Slot& s : pl.slot) {
     SBWT_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_chars), chars_cap, hipHostMallocDefault));
     SBWT_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_starts), rows_cap * 8, hipHostMallocDefault));
-    if ((stats && output != SBWT_OUTPUT_BITMAP) || output == SBWT_OUTPUT_NODES)
+    if (output == SBWT_OUTPUT_NODES)
       SBWT_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_out), chars_cap * 8, hipHostMallocDefault));
-    if (bits_cap) {
+    if (output == SBWT_OUTPUT_BITMAP)
       SBWT_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_bits), bits_cap * 8, hipHostMallocDefault));
-      SBWT_HIP(hipMalloc(&s.d_bits, bits_cap * 8));
+    if (bits_cap) SBWT_HIP(hipMalloc(&s.d_bits, bits_cap * 8));
+    if (stats) {
+      SBWT_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_found), 8, hipHostMallocDefault));
+      SBWT_HIP(hipMalloc(&s.d_found, 8));
     }
     if (text_out) {
       SBWT_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_text), text_cap, hipHostMallocDefault));
@@ -184,10 +203,7 @@ extern "C" int sbwt_search_text(const sbwt_index* ix, const char* text, uint64_t
     SBWT_HIP(hipEventSynchronize(s.returned));
     s.busy = false;
     const u64 words = (s.kmers + 63) / 64;
-    if (stats && s.h_bits)
-      for (u64 w = 0; w < words; w++) total_found += __builtin_popcountll(s.h_bits[w]);
-    else if (stats)
-      for (u64 q = 0; q < s.kmers; q++) total_found += s.h_out[q] != kNone;
+    if (stats) total_found += *s.h_found;
     if (text_out) {
       const u64 size = *s.h_size;
       if (size > s.text_bytes) return fail(SBWT_ERR_HIP, "result text of %llu bytes exceeds its %llu-byte buffer",
@@ -206,7 +222,10 @@ extern "C" int sbwt_search_text(const sbwt_index* ix, const char* text, uint64_t
     Slot& s = pl.slot[b & 1];
     SBWT_TRY(finish(s));  // batch b - 2
     u64 rows = 0;
-    SBWT_TRY(parse_text(kind, text + cut[b], cut[b + 1] - cut[b], s.h_chars, s.h_starts, rows_cap, &rows));
+    if (kind == kTextLines)
+      SBWT_TRY(parse_lines_parallel(text + cut[b], cut[b + 1] - cut[b], s.h_chars, s.h_starts, rows_cap, &rows));
+    else
+      SBWT_TRY(parse_text(kind, text + cut[b], cut[b + 1] - cut[b], s.h_chars, s.h_starts, rows_cap, &rows));
     const u64 n_chars = s.h_starts[rows];
     s.rows = rows;
     s.offsets.assign(rows + 1, 0);
@@ -231,9 +250,8 @@ extern "C" int sbwt_search_text(const sbwt_index* ix, const char* text, uint64_t
       s.text_bytes = s.kmers * width + rows;
       SBWT_TRY(sbwt_format_text(s.d_out, s.d_offsets, rows, s.kmers, s.d_text, s.text_bytes, s.d_size,
                                 s.d_scratch, scratch, pl.comp));
-    } else if (s.d_bits) {
-      SBWT_TRY(sbwt_found_bitmap(s.d_out, s.kmers, 0, s.d_bits, nullptr, pl.comp));
     }
+    if (s.d_bits) SBWT_TRY(sbwt_found_bitmap(s.d_out, s.kmers, 0, s.d_bits, s.d_found, pl.comp));
     SBWT_HIP(hipEventRecord(s.searched, pl.comp));
     SBWT_HIP(hipStreamWaitEvent(pl.d2h, s.searched, 0));
     if (text_out) {
@@ -242,6 +260,7 @@ extern "C" int sbwt_search_text(const sbwt_index* ix, const char* text, uint64_t
     }
     if (s.h_bits && s.kmers)
       SBWT_HIP(hipMemcpyAsync(s.h_bits, s.d_bits, (s.kmers + 63) / 64 * 8, hipMemcpyDeviceToHost, pl.d2h));
+    if (s.h_found) SBWT_HIP(hipMemcpyAsync(s.h_found, s.d_found, 8, hipMemcpyDeviceToHost, pl.d2h));
     if (s.h_out && s.kmers)
       SBWT_HIP(hipMemcpyAsync(s.h_out, s.d_out, s.kmers * 8, hipMemcpyDeviceToHost, pl.d2h));
     SBWT_HIP(hipEventRecord(s.returned, pl.d2h));

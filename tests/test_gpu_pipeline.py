@@ -84,6 +84,26 @@ def test_lines_text_equals_one_shot(index, tmp_path, batch):
     assert stats == {"rows": len(starts) - 1, "kmers": kmers, "found": found}
 
 
+def test_large_batches_parse_on_threads(index, tmp_path):
+    """Batches of 4 MiB and more are parsed on host threads and their found
+    k-mers counted on the device: the text and the counts still equal the
+    one-shot path's."""
+    ix, g = index
+    rng = np.random.default_rng(99)
+    lines = _reads(rng, g, 30_000, 100, 300) + [">h", "", "ACGT\r"] + _reads(rng, _genome(rng, 60_000), 10_000, 100, 300)
+    rng.shuffle(lines)
+    text = ("\n".join(lines) + "\n").encode()
+    assert len(text) > (6 << 20)
+    chars, starts = O.parse_lines(text)
+    want, kmers, found = _expected(ix, chars, starts)
+    for batch in (0, 5 << 20):
+        got, stats = _run(ix, text, tmp_path, batch_chars=batch)
+        assert got == want
+        assert stats == {"rows": len(starts) - 1, "kmers": kmers, "found": found}
+        got, stats = _run(ix, text, tmp_path, batch_chars=batch, output="bitmap")
+        assert stats["found"] == found
+
+
 @pytest.mark.parametrize("fmt", ["fasta", "fastq"])
 def test_records_text_equals_one_shot(index, tmp_path, fmt):
     ix, g = index

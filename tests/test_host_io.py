@@ -42,6 +42,47 @@ def test_parse_lines_matches_oracle(text):
     assert list(map(int, r1)) == list(map(int, r2))
 
 
+@pytest.mark.parametrize("seed", [0, 1])
+def test_parse_lines_threads_match_oracle(seed):
+    """Texts of 4 MiB and more are parsed on host threads, in pieces cut at
+    line starts: the rows equal the oracle's, whatever lands on a cut
+    ('>' lines, empty lines, CRLF, a last line without a newline), and a
+    row capacity too small is still refused."""
+    import ctypes as C
+
+    import sbwt_gpu_amd as sg
+    from sbwt_gpu_amd import _capi
+
+    rng = np.random.default_rng(seed)
+    alphabet = np.frombuffer(b"ACGTNacgt", np.uint8)
+    lines = []
+    size = 0
+    while size < (6 << 20):
+        kind = rng.random()
+        if kind < 0.02:
+            ln = b">header " + bytes(alphabet[rng.integers(0, 9, 20)])
+        elif kind < 0.04:
+            ln = b""
+        else:
+            ln = bytes(alphabet[rng.integers(0, 9, int(rng.integers(1, 400)))])
+        if rng.random() < 0.1:
+            ln += b"\r"
+        lines.append(ln)
+        size += len(ln) + 1
+    text = b"\n".join(lines) + (b"\n" if seed else b"")
+    c1, r1 = sg.parse_lines(text)
+    c2, r2 = O.parse_lines(text)
+    assert bytes(c1) == bytes(c2)
+    np.testing.assert_array_equal(np.asarray(r1, np.uint64), np.asarray(r2, np.uint64))
+    L = _capi.load()
+    rows = C.c_uint64()
+    chars = np.zeros(len(text), np.uint8)
+    rs = np.zeros(len(r2) - 1, np.uint64)  # one entry short
+    with pytest.raises(sg.SbwtError, match="more than"):
+        _capi.check(L.sbwt_parse_lines(text, len(text), C.c_void_p(chars.ctypes.data), C.c_void_p(rs.ctypes.data),
+                                       rs.size, C.byref(rows)))
+
+
 def test_format_results_matches_oracle():
     import sbwt_gpu_amd as sg
 
