@@ -139,3 +139,22 @@ def test_damaged_images_are_refused(tmp_path, small):
     with pytest.raises(sg.SbwtError, match="can not be opened"):
         sg.Index.load(str(tmp_path / "missing.img"))
     ix.close()
+
+
+@pytest.mark.parametrize("k", [9, 15, 32])
+def test_image_round_trip_k(tmp_path, k):
+    """Images of indexes at the edges of k (short k-mers, the largest k)
+    reload with their k and answer as before."""
+    rng = np.random.default_rng(k)
+    g = genome(rng, 40_000)
+    s = O.SBWT([g], k)
+    ix = sg.Index.from_rows(s.rows, s.n, k, group=s.group, config=sg.Config(presearch_depth=min(k, 8)))
+    path = str(tmp_path / "k.img")
+    ix.save(path)
+    back = sg.Index.load(path)
+    assert back.k == k
+    _same_index(ix, back)
+    batch = sg.ReadBatch.from_host([g[100:400], g[5000:5100] + "ACGTTGCA" + g[7000:7100]], k)
+    np.testing.assert_array_equal(sg.to_u64(back.search_reads(batch)), sg.to_u64(ix.search_reads(batch)))
+    ix.close()
+    back.close()

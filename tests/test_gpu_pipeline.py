@@ -272,3 +272,29 @@ def test_found_bitmap(count):
         want = np.packbits(padded, bitorder="little").view(np.uint64)
         np.testing.assert_array_equal(sg.to_u64(bits), want)
         assert int(found.item()) == int((res != sg.NONE).sum())
+
+
+@pytest.mark.parametrize("output", ["nodes", "bitmap"])
+def test_binary_outputs_of_records(index, tmp_path, output):
+    """FASTQ records (one row each) with binary outputs: row counts and
+    values equal the one-shot search of the parsed records."""
+    ix, g = index
+    rng = np.random.default_rng(21)
+    parts = []
+    for i, r in enumerate(_reads(rng, g, 300, 20, 300)):
+        w = int(rng.integers(20, 80))
+        wrap = [r[j:j + w] for j in range(0, len(r), w)]
+        parts += [f"@r{i}"] + wrap + ["+"] + ["I" * len(x) for x in wrap]
+    text = ("\n".join(parts) + "\n").encode()
+    chars, starts = R.parse_records(text)
+    batch = sg.ReadBatch(torch.from_numpy(np.ascontiguousarray(chars, np.uint8).copy()).cuda(),
+                         torch.from_numpy(np.asarray(starts, np.int64).copy()).cuda(), 31,
+                         host_starts=np.asarray(starts, np.int64))
+    want = sg.to_u64(ix.search_reads(batch))
+    lens = np.diff(np.asarray(starts, np.int64))
+    for b in (0, 2_000):
+        got, stats = _run(ix, text, tmp_path, records=True, batch_chars=b, output=output)
+        dec = sg.read_binary_results(got)
+        np.testing.assert_array_equal(dec["row_kmers"], np.maximum(lens - 30, 0))
+        np.testing.assert_array_equal(dec["values"], want if output == "nodes" else want != sg.NONE)
+        assert stats["found"] == int((want != sg.NONE).sum())
