@@ -307,6 +307,44 @@ def test_trained_prefilter_zero_false_negatives_and_exact_answers():
     assert m[plain == NONE].mean() < 0.9
 
 
+def test_learned_filter_gains_on_composition():
+    """What the trained network gains over the filter without it, where
+    there is something to learn: an index of a GC-rich genome queried with
+    reads of an AT-rich one. The network learns the composition, so its
+    Bloom filter only holds the indexed k-mers it rejects. The learned
+    filter (network + that Bloom filter) keeps zero false negatives, passes
+    few of the AT-rich k-mers, and is several times smaller than a Bloom
+    filter over every indexed k-mer at the same false-positive rate. On
+    uniform genomes there is no such signal (the network stays at chance)."""
+    rng = np.random.default_rng(33)
+    k = 31
+
+    def biased(n, gc):
+        p = [(1 - gc) / 2, gc / 2, gc / 2, (1 - gc) / 2]
+        return "".join(ACGT[rng.choice(4, size=n, p=p)])
+
+    g = biased(300_000, 0.72)
+    ix = sg.Index.build([g], k, sg.Config(presearch_depth=10))
+    pf = train_prefilter(ix, positives=250_000, negatives=250_000, batch=4096, epochs=3, lr=2e-3, seed=5)
+    assert pf.stats["zero_false_negatives_on_index"]
+    allk = ix.enumerate_kmers()
+    assert bool((pf.maybe(allk) == 1).all().item())
+    at_reads = sg.ReadBatch.from_host(_reads(rng, biased(200_000, 0.28), 2_000, 150), k)
+    km = at_reads.kmers()
+    assert bool((ix.search_kmers(km) == -1).all().item())  # no AT-rich k-mer is indexed
+    learned_pass = float(pf.maybe(km).float().mean().item())
+    whole = sg.Bloom(k, allk.numel(), 0.01)
+    whole.insert(allk)
+    whole_pass = float(whole.query(km).float().mean().item())
+    learned_bytes = pf.stats["bloom_bytes"] + pf.mlp.param_count * 2  # bf16 weights
+    print(f"learned filter: {learned_bytes} B, passes {learned_pass:.4f} of AT-rich k-mers; "
+          f"Bloom over every indexed k-mer: {whole.bytes} B, passes {whole_pass:.4f}")
+    assert learned_pass < 0.1, learned_pass
+    assert learned_bytes * 2 < whole.bytes, (learned_bytes, whole.bytes)
+    assert whole_pass < 0.05
+    whole.close()
+
+
 def test_masked_kmer_search_equals_plain_where_maybe():
     rng = np.random.default_rng(21)
     g = _genome(rng, 50_000)
