@@ -472,13 +472,22 @@ __device__ __forceinline__ u32 select_bit(u64 m, u32 n) {
 }
 
 // Once the work items run out, lanes of a wave that have nothing left take
-// the second half of the k-mers left to lanes with at least kSplitMin of
-// them (pairs in lane order). A lane's last item would otherwise run alone
-// while the rest of its wave idles. The half starts with a search of about
-// k - m steps, so a split pays when the half is longer: on the 100 Mbp read
-// batch the kernel takes 4.33 ms without splits, 4.21 ms with kSplitMin 32
-// or 48, 4.25 ms with 16.
+// the second half of the k-mers left to other lanes (pairs in lane order).
+// A lane's last item would otherwise run alone while the rest of its wave
+// idles. The half starts with a search of about k - m steps, so a split pays
+// when the donor's remaining work is longer than that:
+// * a streaming lane spends about one step per k-mer: it gives from
+//   kSplitMin k-mers on (on the 100 Mbp read batch the kernel took 4.33 ms
+//   without splits, 4.21 ms with kSplitMin 32 or 48, 4.25 ms with 16);
+// * a lane searching its k-mer from scratch (the previous one was absent and
+//   no certificate covered this one: an error in a row's first characters,
+//   a certificate search that ran into a chance match) may need a whole
+//   search for each k-mer left: it gives from kSplitSearchMin on. These lanes
+//   set the length of a wave's tail. With them the batch's lane-iterations
+//   fall from 209 to 171 per read (151 to 159 busy) and the kernel from 3.61
+//   to 3.41 ms; 2, 4 and 8 measure alike.
 constexpr u32 kSplitMin = 48;
+constexpr u32 kSplitSearchMin = 4;
 
 template <class W>
 __device__ __forceinline__ void donate(W& w, const ReadsDev& rd, u32 lane) {
@@ -486,7 +495,8 @@ __device__ __forceinline__ void donate(W& w, const ReadsDev& rd, u32 lane) {
   const u64 im = __ballot(idle);
   if (!im) return;
   const u32 rem = w.live && w.j < w.jend ? w.jend - w.j - 1 : 0u;  // after the k-mer at j
-  const bool give = rem >= kSplitMin;
+  const bool searching = !w.prev && w.mode == M_SEARCH && w.purpose == P_KMER;
+  const bool give = rem >= kSplitMin || (searching && rem >= kSplitSearchMin);
   const u64 gm = __ballot(give);
   if (!gm) return;
   const u32 ni = static_cast<u32>(__popcll(im)), ng = static_cast<u32>(__popcll(gm));
