@@ -270,6 +270,41 @@ def test_search_reads_errors_near_row_end(layout):
     assert searches[(("lanes", True),)] < searches[(("certificates", False), ("lanes", True))]
 
 
+@pytest.mark.parametrize("e", [3, 75, 135])
+def test_search_reads_same_error_position(e):
+    """Every read holds one substitution at the same position, in a genome
+    with segmental copies at 1%. The lanes of a wave reach their errors
+    together; lanes whose certificate search runs into a copy search their
+    k-mers one by one, and lanes that run out of work split those lanes'
+    k-mers between them. Every read kernel answers as the oracle."""
+    rng = np.random.default_rng(300 + e)
+    base = genome(rng, 60_000)
+    parts = [base]
+    for _ in range(40):
+        p = int(rng.integers(0, 59_000))
+        seg = np.array(list(base[p:p + 1000]))
+        flip = rng.random(1000) < 0.01
+        seg[flip] = ACGT[rng.integers(0, 4, flip.sum())]
+        parts.append("".join(seg))
+    g = "".join(parts)
+    k = 31
+    s = O.SBWT([g], k)
+    reads = []
+    for _ in range(3000):
+        p = int(rng.integers(0, len(g) - 150))
+        r = np.array(list(g[p:p + 150]))
+        r[e] = "ACGT"[("ACGT".index(r[e]) + int(rng.integers(1, 4))) % 4]
+        reads.append("".join(r))
+    chars, rs = O.parse_lines("\n".join(reads).encode())
+    want = O.search_reads(oracle_index(s, 10), chars, rs, k)
+    assert (want == np.uint64(~np.uint64(0))).sum() >= 3000  # each read's k-mers over the error are absent
+    ix = sg.Index.from_rows(s.rows, s.n, k, group=s.group, config=sg.Config(presearch_depth=10))
+    batch = sg.ReadBatch(torch.from_numpy(np.frombuffer(bytes(chars), np.uint8).copy()).cuda(), dev_u64(rs), k,
+                         seg=256, host_starts=rs)
+    for kw in READ_VARIANTS:
+        np.testing.assert_array_equal(sg.to_u64(ix.search_reads(batch, **kw)), want, err_msg=str(kw))
+
+
 @pytest.mark.parametrize("negative", [0.1, 0.5, 0.9])
 def test_search_reads_mixed_negative_reads(negative):
     """Batches mixing reads from the index's genome (with substitutions,
