@@ -306,13 +306,17 @@ __global__ __launch_bounds__(256) void k_search_reads(DevIndex ix, const u64* __
           }
         }
       } else if (CERT && d >= rowk && !(clo <= chi && chi > j)) {
-        // no k-mer starts at d; the k-mers j+1 .. q all contain [q, d], a
-        // string of dlen characters
+        // no k-mer starts at d; the k-mers j+1 .. q all contain [s, d] for
+        // any s >= q: a string of dlen characters, or of dlen + 2 when the
+        // row's last k-mer starts further from d
         const u64 dl = dlen;
         const u64 q = d + 1 >= dl ? (rowk - 1 < d + 1 - dl ? rowk - 1 : d + 1 - dl) : 0;
+        const u64 ls = min(k, dlen + 2);
+        const u64 s0 = d + 1 >= ls ? d + 1 - ls : 0;
+        const u64 sq = s0 > q ? s0 : q;
         u32 dd;
         if (d + 1 >= dl && q > j &&
-            search_len<CB, COUNT>(ix, extract2(packed, rs + q), static_cast<u32>(d - q + 1), t, dd) == kNone) {
+            search_len<CB, COUNT>(ix, extract2(packed, rs + sq), static_cast<u32>(d - sq + 1), t, dd) == kNone) {
           clo = j + 1;
           chi = q;
         }

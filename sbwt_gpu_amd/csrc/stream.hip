@@ -214,15 +214,20 @@ struct Walker {
     if (d < j) return;                   // nothing ahead to certify
     if (d >= rowk) {
       // No k-mer starts at d (d is in the row's last k - 1 characters). The
-      // k-mers j .. q, q <= rowk-1, all contain [q, d] (each ends after d).
-      // Far from d their searches are long (each runs to d); q is the last
-      // start whose string [q, d] is long enough to be absent from the
-      // index by chance, so one short search can answer them all.
+      // k-mers j .. q, q <= rowk-1, all contain [s, d] for any s >= q (each
+      // starts at or before s and ends after d). Far from d their searches
+      // are long (each runs to d); q is the last start whose string [q, d]
+      // is long enough to be absent from the index by chance, so one short
+      // search can answer them all. When the row's last k-mer starts more
+      // than dead_len + 2 characters before d, the string starts at
+      // d + 1 - (dead_len + 2) instead: as certain, and shorter.
       const u32 q = min(rowk - 1, d + 1 - dead_len(ix));
       if (rowk == 0 || d + 1 < dead_len(ix) || q < j) return;
+      const u32 ls = min(ix.k, dead_len(ix) + 2);
+      const u32 s = max(q, d + 1 >= ls ? d + 1 - ls : 0u);
       chi = q;
       clo = q + 1;  // empty until the search proves it (P_CERTL then sets clo = j)
-      start_search(ix, rd, t, P_CERTL, q, d - q + 1);
+      start_search(ix, rd, t, P_CERTL, s, d - s + 1);
       return;
     }
     // The search from d certifies only if it dies, which a string starting
