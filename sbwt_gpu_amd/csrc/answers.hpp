@@ -4,12 +4,19 @@
 // the index reads streaming through the L2, a partially written output line
 // is evicted before the lane comes back to it, and is written back piecemeal
 // (measured: 3.3 GB of HBM writes per 0.96 GB of answers). So each lane
-// collects its answers per aligned group of four output slots (32 bytes) in
+// collects its answers per aligned group of S output slots (S * 8 bytes) in
 // LDS, slot-major (slot s of thread t at obuf[s * kAnswerStride + t], so a
-// wave's accesses never conflict), and a full group goes out as two 16-byte
-// stores; partial groups (row and item edges) go out slot by slot. Runs of
-// one value (k-mers answered ~0 without a search) skip the buffer for their
-// whole groups.
+// wave's accesses never conflict), and a full group goes out as S / 2
+// 16-byte stores; partial groups (row and item edges) go out slot by slot.
+// Runs of one value (k-mers answered ~0 without a search) skip the buffer for
+// their whole groups.
+//
+// S = 4 (32-byte groups) by default. The uniform-step kernel on 16-byte
+// chunks uses S = 8 (64 bytes, 16 KB of LDS per 256-thread block): the
+// answers' writes still cost that kernel about 0.2 ms per 1M-read batch
+// (3.18 ms with the same stores sent to an L2-resident region), and 64-byte
+// groups take back some of it (3.39 -> 3.36 ms; 2.59 -> 2.50 ms on a 1 Mbp
+// index). Non-temporal stores were slower (3.45 ms at S = 4, 3.53 at S = 8).
 #pragma once
 #include "common.hpp"
 
@@ -17,73 +24,75 @@ namespace sbwt {
 
 constexpr u32 kAnswerStride = 256;  // threads per block of the kernels that use it
 
-struct AnswerBuffer {
+template <u32 S>
+struct AnswerBufferT {
+  static_assert(S >= 2 && S <= 16 && (S & (S - 1)) == 0, "a power of two, whole 16-byte stores");
   u64* obuf;  // this thread's slot 0 in LDS
   u64 group = ~0ull;
   u32 mask = 0;
 
   __device__ __forceinline__ void flush(u64* __restrict__ out, bool vec) {
     if (!mask) return;
-    u64* o = out + group * 4;
-    const u64 b0 = obuf[0], b1 = obuf[kAnswerStride], b2 = obuf[2 * kAnswerStride],
-              b3 = obuf[3 * kAnswerStride];
-    if (mask == 15 && vec) {
-      reinterpret_cast<uint4*>(o)[0] = make_uint4(static_cast<u32>(b0), static_cast<u32>(b0 >> 32),
-                                                  static_cast<u32>(b1), static_cast<u32>(b1 >> 32));
-      reinterpret_cast<uint4*>(o)[1] = make_uint4(static_cast<u32>(b2), static_cast<u32>(b2 >> 32),
-                                                  static_cast<u32>(b3), static_cast<u32>(b3 >> 32));
+    u64* o = out + group * S;
+    if (mask == (1u << S) - 1 && vec) {
+#pragma unroll
+      for (u32 h = 0; h < S / 2; h++) {
+        const u64 x = obuf[(2 * h) * kAnswerStride], y = obuf[(2 * h + 1) * kAnswerStride];
+        reinterpret_cast<uint4*>(o)[h] = make_uint4(static_cast<u32>(x), static_cast<u32>(x >> 32),
+                                                    static_cast<u32>(y), static_cast<u32>(y >> 32));
+      }
     } else {
-      if (mask & 1) o[0] = b0;
-      if (mask & 2) o[1] = b1;
-      if (mask & 4) o[2] = b2;
-      if (mask & 8) o[3] = b3;
+#pragma unroll
+      for (u32 sl = 0; sl < S; sl++)
+        if (mask & (1u << sl)) o[sl] = obuf[sl * kAnswerStride];
     }
     mask = 0;
   }
 
   // out[q] = v; `last`: the lane's item ends at q. A lane answers its items'
   // k-mers in order, so the buffer only ever holds the group of q - 1 (and is
-  // empty at a group start): a group goes out when its slot 3 is written or
-  // its item ends.
+  // empty at a group start): a group goes out when its last slot is written
+  // or its item ends.
   __device__ __forceinline__ void put(u64* __restrict__ out, bool vec, u64 q, u64 v, bool last) {
-    const u32 sl = static_cast<u32>(q) & 3u;
+    const u32 sl = static_cast<u32>(q) & (S - 1);
     obuf[sl * kAnswerStride] = v;
     mask |= 1u << sl;
-    group = q >> 2;
-    if (sl == 3 || last) flush(out, vec);
+    group = q / S;
+    if (sl == S - 1 || last) flush(out, vec);
   }
 
   // out[q .. q+n-1] = v (n >= 1); `last`: the lane's item ends at q+n-1.
   // Whole aligned groups go straight from registers to memory.
   __device__ __forceinline__ void put_run(u64* __restrict__ out, bool vec, u64 q, u64 n, u64 v, bool last) {
     do {
-      const u32 sl = static_cast<u32>(q) & 3u;
-      if (sl == 0 && n >= 4) {
+      const u32 sl = static_cast<u32>(q) & (S - 1);
+      if (sl == 0 && n >= S) {
         u64* o = out + q;
         if (vec) {
           const uint4 vv = make_uint4(static_cast<u32>(v), static_cast<u32>(v >> 32), static_cast<u32>(v),
                                       static_cast<u32>(v >> 32));
-          reinterpret_cast<uint4*>(o)[0] = vv;
-          reinterpret_cast<uint4*>(o)[1] = vv;
+#pragma unroll
+          for (u32 h = 0; h < S / 2; h++) reinterpret_cast<uint4*>(o)[h] = vv;
         } else {
-          o[0] = v;
-          o[1] = v;
-          o[2] = v;
-          o[3] = v;
+#pragma unroll
+          for (u32 h = 0; h < S; h++) o[h] = v;
         }
-        q += 4;
-        n -= 4;
+        q += S;
+        n -= S;
       } else {
         obuf[sl * kAnswerStride] = v;
         mask |= 1u << sl;
-        group = q >> 2;
+        group = q / S;
         q++;
         n--;
-        if (sl == 3 || (last && n == 0)) flush(out, vec);
+        if (sl == S - 1 || (last && n == 0)) flush(out, vec);
       }
     } while (n);
   }
 };
+
+constexpr u32 kAnswerSlots = 4;
+using AnswerBuffer = AnswerBufferT<kAnswerSlots>;
 
 // Prefilter flags (one byte per k-mer, nonzero = search it) read by a lane
 // walking its own k-mers in order. Thousands of lanes per CU each walk their

@@ -213,7 +213,7 @@ __global__ __launch_bounds__(256) void k_search_reads(DevIndex ix, const u64* __
                                                       const u64* __restrict__ work, u64 rows, u64 bound, u32 seg,
                                                       const u8* __restrict__ maybe, u64* __restrict__ out,
                                                       u32 vec_out, unsigned long long* ctr) {
-  __shared__ u64 obuf[4 * kAnswerStride];
+  __shared__ u64 obuf[kAnswerSlots * kAnswerStride];
   AnswerBuffer ab{obuf + threadIdx.x};
   const u64 gid = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x;
   Tally t;

@@ -82,9 +82,18 @@ struct UTally {
 };
 
 // The state of one work item being walked by one lane.
+// Answer slots per lane: 64-byte groups for the 16-byte chunk layout (the
+// larger chunks' kernels hold more registers, and a larger group costs them
+// a wave of occupancy).
+template <int CB>
+constexpr u32 answer_slots() {
+  return CB == 16 ? 8u : kAnswerSlots;
+}
+
 template <int CB, bool COUNT>
 struct Walker {
   static constexpr u32 COLS = Layout<CB>::COLS;
+  using Ans = AnswerBufferT<answer_slots<CB>()>;
   // the item: k-mers [j, jend) of a row of rowk k-mers starting at character cbase
   u64 cbase = 0, obase = 0;
   u32 j = 0, jend = 0, rowk = 0;
@@ -182,7 +191,7 @@ struct Walker {
   }
 
   // the answer for the k-mer at j (through the lane's answer buffer); moves to j + 1
-  __device__ __forceinline__ void answer(u64* __restrict__ out, const ReadsDev& rd, AnswerBuffer& ab, UTally& t,
+  __device__ __forceinline__ void answer(u64* __restrict__ out, const ReadsDev& rd, Ans& ab, UTally& t,
                                          u64 v) {
     ab.put(out, rd.vec_out != 0, obase + j, v, j + 1 >= jend);
     j++;
@@ -190,7 +199,7 @@ struct Walker {
   }
 
   // v for the n k-mers from j on; moves to j + n
-  __device__ __forceinline__ void answer_run(u64* __restrict__ out, const ReadsDev& rd, AnswerBuffer& ab,
+  __device__ __forceinline__ void answer_run(u64* __restrict__ out, const ReadsDev& rd, Ans& ab,
                                              UTally& t, u32 n, u64 v) {
     ab.put_run(out, rd.vec_out != 0, obase + j, n, v, j + n >= jend);
     j += n;
@@ -264,7 +273,7 @@ struct Walker {
   // Answer what needs no index access, then set up one step.
   // (nk: the number of k-mers of the batch, the length of rd.maybe)
   __device__ __forceinline__ void advance(const DevIndex& ix, const ReadsDev& rd, u64 nk, u64* __restrict__ out,
-                                          AnswerBuffer& ab, UTally& t) {
+                                          Ans& ab, UTally& t) {
     const u32 k = ix.k;
     while (mode == M_IDLE && j < jend) {
       // a run of k-mers answered ~0 without a search: certified absent,
@@ -330,7 +339,7 @@ struct Walker {
   // wave's recent P_CERTS searches mostly ended alive); returns +1 / -1 when
   // a P_CERTS search ends alive / dead, else 0.
   __device__ __forceinline__ int update(const DevIndex& ix, const ReadsDev& rd, u64* __restrict__ out,
-                                        AnswerBuffer& ab, UTally& t, const Chunk<CB>& qa, const Chunk<CB>& qb,
+                                        Ans& ab, UTally& t, const Chunk<CB>& qa, const Chunk<CB>& qb,
                                         bool skip_short) {
     const u32 k = ix.k;
     const Cols st = cols(ix);
@@ -528,8 +537,8 @@ __global__ __launch_bounds__(256) void k_reads_uniform(DevIndex ix, ReadsDev rd,
                                                        u64* __restrict__ out, unsigned long long* ctr) {
   using W = Walker<CB, COUNT>;
   static_assert(P == 1, "one LDS answer buffer per lane");
-  __shared__ u64 obuf[4 * kAnswerStride];
-  AnswerBuffer ab{obuf + threadIdx.x};
+  __shared__ u64 obuf[answer_slots<CB>() * kAnswerStride];
+  typename W::Ans ab{obuf + threadIdx.x};
   const u32 lane = threadIdx.x & 63;
   const u64 total_items = rd.rows ? rd.work[rd.rows] : 0;
   const u64 nk = rd.maybe && rd.rows ? rd.offsets[rd.rows] : 0;
