@@ -235,6 +235,11 @@ def main() -> int:
         roof = gather_roof()
         roofline["measured_random_roof"] = roof
         roofline["frac_of_measured_roof"] = round(achieved / roof["GBps_128B_lines"], 4)
+        # the same gather over a buffer the size of the index (rows, chunks and
+        # table), which the last-level cache partly holds, as the search's does
+        ws = gather_roof(nbytes=int(info["device_bytes"]))
+        roofline["working_set_random_roof"] = ws
+        roofline["frac_of_working_set_roof"] = round(achieved / ws["GBps_128B_lines"], 4)
 
     # ---- CPU baseline + parity on a sample -----------------------------------
     cpu = None
@@ -1221,25 +1226,26 @@ def traffic_bytes(t):
     return t["counters"]["FETCH_SIZE"]["bytes"] + t["counters"]["WRITE_SIZE"]["bytes"]
 
 
-def gather_roof(gib: int = 4):
+def gather_roof(gib: int = 4, nbytes: int | None = None):
     """Uniform-random 128-byte-line gathers over a `gib` GiB buffer (far
     beyond the 256 MiB last-level cache): the random-access roof. Two shapes:
     one 16-byte read per random line (the search's access), and whole lines
-    read by 8 lanes together."""
+    read by 8 lanes together. `nbytes` (a multiple of 128) replaces the size:
+    the roof for a working set of that size, which the caches partly hold."""
     import torch
 
     from sbwt_gpu_amd import _capi
     from sbwt_gpu_amd.api import _ptr, _stream
 
     L = _capi.load()
-    nbytes = gib << 30
+    nbytes = (gib << 30) if nbytes is None else max(128, nbytes // 128 * 128)
     buf = torch.empty(nbytes // 8, dtype=torch.int64, device="cuda")
     buf.fill_(0x5A5A)
     lanes, per = 256 * 1024 * 8, 64
     sink = torch.empty(lanes, dtype=torch.int64, device="cuda")
     s = _stream()
     st = torch.cuda.current_stream()
-    out = {"buffer_GiB": gib}
+    out = {"buffer_GiB": round(nbytes / (1 << 30), 3)}
     for name, full in (("one_16B_read_per_line", 0), ("whole_line_8_lanes", 1)):
         _capi.check(L.sbwt_gather_roof(_ptr(buf), nbytes, 128, full, per, lanes, 7, _ptr(sink), s))
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
