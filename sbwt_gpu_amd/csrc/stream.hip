@@ -29,7 +29,9 @@
 // wave execute the same instructions whatever they are doing, instead of
 // one lane running a k-step search while the others wait. Lanes take new
 // work items (segments of rows) from their wave's pool, which claims 64
-// items at a time from a global counter.
+// items at a time from a global counter; once the counter runs out, lanes
+// left without work take half of what other lanes of their wave still hold
+// (donate()).
 #include "answers.hpp"
 #include "index.hpp"
 
