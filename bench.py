@@ -490,6 +490,8 @@ def run_batched(args) -> int:
     if not args.no_autotune:
         sample_chars, sample_rel, _, _ = batches[0]
         t = time.time()
+        # depths up to 13: deeper tables win the trials (repeated searches of
+        # one batch, whose table entries stay cached) but lose over the pass
         tune = autotune.autotune(ix, sample_chars, sample_rel, segs=(64, 256, 1024), budget_s=90.0)
         seg = tune["best"]["segment_kmers"]
         persistent = tune["best"]["kernel"] == "persistent"
@@ -1099,7 +1101,7 @@ def cpu_baseline_rows(ix, chars, starts_h, res_gpu, k, args):
 
     rows = ix.rows()
     oix = O.Index(rows[:4], ix.n, sb=1024)
-    oix.presearch(ix.info["presearch_depth"])
+    oix.presearch(min(ix.info["presearch_depth"], 12))  # answers do not depend on the depth
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
     nrows = min(len(starts_h) - 1, 200_000)
     c_end = int(starts_h[nrows])
