@@ -32,6 +32,8 @@ extern "C" {
 #define SBWT_ERR_IO 3          /* file missing / short */
 #define SBWT_ERR_FORMAT 4      /* inconsistent index (unequal rows, C[4] != n) */
 #define SBWT_ERR_UNSUPPORTED 5 /* configuration this build does not support */
+#define SBWT_ERR_INTERNAL 6    /* a device-side consistency check failed; the
+                                  message names the stage */
 
 /* ---- rank layouts: bytes per rank chunk (u32 count + payload bits) */
 #define SBWT_LAYOUT_AUTO 0
@@ -110,6 +112,13 @@ int sbwt_index_from_files(const char *const paths[4], const char *group_path,
 int sbwt_index_build(const uint8_t *d_seq, const uint64_t *d_starts,
                      uint64_t nseq, uint32_t k, const sbwt_config *cfg,
                      void *stream, sbwt_index **out);
+
+/* Largest item count of one sort, select or unique call inside
+ * sbwt_index_build (0 = the default, 2^28); larger buckets are split by
+ * further key bits. Process-wide; not for concurrent builds. It exists so
+ * tests can exercise the bucket seams on small inputs. No reference
+ * counterpart. */
+int sbwt_build_set_sort_chunk(uint64_t max_items);
 
 /* Frees every device buffer (include/gpu/device_memory.hpp:81-121). */
 int sbwt_index_free(sbwt_index *ix);

@@ -10,7 +10,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsbwt_gpu_amd.so")
 
 SBWT_OK = 0
-ERRORS = {1: "SBWT_ERR_ARG", 2: "SBWT_ERR_HIP", 3: "SBWT_ERR_IO", 4: "SBWT_ERR_FORMAT", 5: "SBWT_ERR_UNSUPPORTED"}
+ERRORS = {1: "SBWT_ERR_ARG", 2: "SBWT_ERR_HIP", 3: "SBWT_ERR_IO", 4: "SBWT_ERR_FORMAT", 5: "SBWT_ERR_UNSUPPORTED",
+          6: "SBWT_ERR_INTERNAL"}
 READS_N_AS_A = 1
 READS_INDEPENDENT = 2
 COUNT = 4
@@ -44,6 +45,7 @@ SIGNATURES = {
     "sbwt_index_from_rows": (i32, [u64p * 4, vp, u64, u32, C.POINTER(Config), C.POINTER(vp)]),
     "sbwt_index_from_files": (i32, [cp * 4, cp, u32, C.POINTER(Config), C.POINTER(vp)]),
     "sbwt_index_build": (i32, [vp, vp, u64, u32, C.POINTER(Config), vp, C.POINTER(vp)]),
+    "sbwt_build_set_sort_chunk": (i32, [u64]),
     "sbwt_index_free": (i32, [vp]),
     "sbwt_index_get_info": (i32, [vp, C.POINTER(IndexInfo)]),
     "sbwt_index_get_rows": (i32, [vp, u64p * 4, vp, u64]),

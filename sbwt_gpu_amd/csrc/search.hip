@@ -458,8 +458,20 @@ static int search_reads_impl(const sbwt_index* ix, const uint64_t* d_packed, con
   // the uniform-step kernel wins where certificates mix short searches into
   // the streams; the per-lane kernel's tight loop wins for independent
   // searches and streams without certificates
-  const bool uniform = (flags & SBWT_READS_PERSISTENT) ||
-                       (!(flags & SBWT_READS_SEGMENT_LANES) && stream_mode && cert);
+  bool uniform = (flags & SBWT_READS_PERSISTENT) ||
+                 (!(flags & SBWT_READS_SEGMENT_LANES) && stream_mode && cert);
+  // The uniform kernel keeps row-relative positions in u32. A batch whose
+  // work bound admits 2^32 - 64 k-mers or more could hold such a row, so it
+  // goes to the per-lane kernel (u64 positions), or is refused when the
+  // uniform kernel is forced.
+  if (uniform && work_bound > 0xFFFFFFC0ull / seg) {
+    if (flags & SBWT_READS_PERSISTENT)
+      return fail(SBWT_ERR_UNSUPPORTED,
+                  "SBWT_READS_PERSISTENT: a batch of work bound %llu x %u k-mers may hold a row of 2^32 k-mers "
+                  "or more, which the uniform kernel does not support",
+                  (unsigned long long)work_bound, seg);
+    uniform = false;
+  }
   if (uniform)
     return search_reads_persistent(ix, d_packed, d_nmask, d_row_starts, d_offsets, d_work, d_items, rows,
                                    work_bound, seg, d_maybe, d_out, nbreak, stream_mode, cert, cnt,

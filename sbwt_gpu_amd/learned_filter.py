@@ -74,12 +74,17 @@ def train_prefilter(index: Index, *, positives: int = 50_000_000, negatives: int
     t_train = time.time() - t1
     # the model's false negatives over every indexed k-mer go into the Bloom filter
     z = mlp.logits(allpos)
-    fn = allpos[z < threshold]
+    # the exact complement of what the kernel keeps (z >= threshold), so a
+    # NaN logit goes into the Bloom filter too
+    fn = allpos[~(z >= threshold)]
     bloom = Bloom(k, max(fn.numel(), 1), fpr) if fn.numel() else None
     if bloom is not None:
         bloom.insert(fn)
     m = prefilter(mlp, threshold, bloom, allpos)
     zero_fn = bool((m == 1).all().item())
+    if not zero_fn:
+        raise RuntimeError(f"the prefilter rejects {int((m != 1).sum().item())} indexed k-mers; "
+                           "refusing a filter with false negatives")
     zn = mlp.logits(neg[: min(neg.numel(), 10_000_000)])
     stats = {"indexed_kmers": npos_all, "train_positives": pos.numel(), "train_negatives": neg.numel(),
              "epochs": epochs if train else 0, "batch": batch, "lr": lr, "losses": losses,

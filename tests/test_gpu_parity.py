@@ -358,6 +358,26 @@ def test_search_reads_unaligned_output():
         assert int(buf[0]) == 7 and int(buf[-1]) == 7 and int(buf[-2]) == 7
 
 
+def test_uniform_kernel_refuses_batches_that_may_hold_a_4g_row():
+    """The uniform kernel keeps row-relative positions in u32: a forced
+    uniform search whose work bound admits a row of 2^32 k-mers is refused
+    before any launch (the default routes such a batch to the per-lane
+    kernel)."""
+    from sbwt_gpu_amd import _capi
+    from sbwt_gpu_amd.api import _ptr, _stream
+
+    s = O.SBWT(["ACGTACGTTGCAGGCATTACG" * 3], 9)
+    ix = sg.Index.from_rows(s.rows, s.n, 9, group=s.group, config=sg.Config(presearch_depth=4))
+    L = _capi.load()
+    buf = torch.zeros(16, dtype=torch.int64, device="cuda")
+    items = torch.zeros(16, dtype=torch.int32, device="cuda")
+    p = _ptr(buf)
+    st = L.sbwt_search_reads(ix._h, p, p, p, p, p, _ptr(items), 1, 1 << 32, 256, p, _capi.READS_PERSISTENT,
+                             _stream())
+    assert st == 5, L.sbwt_last_error()
+    assert b"2^32" in L.sbwt_last_error()
+
+
 def test_index_without_group_row():
     """Without suffix-group starts every k-mer is searched (certificates still
     apply)."""
