@@ -90,18 +90,23 @@ typedef struct sbwt_index_info {
 int sbwt_config_init(sbwt_config *cfg);
 
 /* Index from four host rows (A,C,G,T; each >= ceil(n/64) words) plus the
- * optional suffix-group-start row (NULL: streaming is unavailable).
- * Replaces epic::MatrixSBWT::create (include/matrix_sbwt.hpp:32-200) after
- * the file read: rank structures, C array, device upload. Fails with
- * SBWT_ERR_FORMAT unless C[4] == n. Builds the presearch table of
- * cfg->presearch_depth. */
+ * optional suffix-group-start row. Replaces epic::MatrixSBWT::create
+ * (include/matrix_sbwt.hpp:32-200) after the file read: rank structures, C
+ * array, device upload. Fails with SBWT_ERR_FORMAT unless C[4] == n. When
+ * group is NULL the row is derived on the device from the four rows (node
+ * d starts a group unless its last k-1 label characters equal node d-1's),
+ * so the streaming read search is available for the reference's own
+ * four-file indexes; a given group row must equal the derived one
+ * (SBWT_ERR_FORMAT otherwise).
+ * Builds the presearch table of cfg->presearch_depth. */
 int sbwt_index_from_rows(const uint64_t *const rows[4], const uint64_t *group,
                          uint64_t n, uint32_t k, const sbwt_config *cfg,
                          sbwt_index **out);
 
 /* Same from the reference's four bit-vector files ([u64 LE n][ceil(n/8)
- * bytes], include/bit_vector.hpp:65-128); group_path may be NULL. Replaces
- * MatrixSBWT::create including BitVector::create/read. */
+ * bytes], include/bit_vector.hpp:65-128); group_path may be NULL (the
+ * group row is then derived, as above). Replaces MatrixSBWT::create
+ * including BitVector::create/read. */
 int sbwt_index_from_files(const char *const paths[4], const char *group_path,
                           uint32_t k, const sbwt_config *cfg, sbwt_index **out);
 

@@ -226,7 +226,7 @@ __global__ void k_clear_tail(u64* rows, u64 nwords, u64 n, u32 mark_root) {
   if (mark_root && r == kGroupRow && w == 0) row[0] |= 1ull << 63;
 }
 
-int finish_index(sbwt_index* ix, hipStream_t s) {
+int finish_index(sbwt_index* ix, hipStream_t s, bool verify) {
   SBWT_HIP(hipMalloc(&ix->d_counters, kCounters * sizeof(unsigned long long)));
   SBWT_HIP(hipMalloc(&ix->d_item_ctr, sizeof(unsigned long long)));
   SBWT_HIP(hipDeviceGetAttribute(&ix->num_cus, hipDeviceAttributeMultiprocessorCount, ix->device));
@@ -261,6 +261,15 @@ int finish_index(sbwt_index* ix, hipStream_t s) {
     return fail(SBWT_ERR_FORMAT,
                 "not an SBWT: C[4] = 1 + total ones = %llu but the rows have n = %llu columns",
                 (unsigned long long)ix->C[4], (unsigned long long)ix->n);
+  // suffix groups: a given row is checked against the edges; a missing one
+  // is derived from the labels, so every index can stream
+  if (ix->has_groups) {
+    SBWT_TRY(verify ? verify_groups(ix, s) : check_groups(ix, s));
+  } else {
+    SBWT_TRY(derive_groups(ix, ix->d_rows + kGroupRow * ix->nwords, s));
+    ix->has_groups = 1;
+    SBWT_TRY(build_layout(ix, s));
+  }
   u32 m = ix->cfg.presearch_depth;
   if (m > ix->k) m = ix->k;
   return run_presearch(ix, m, s);
@@ -340,7 +349,7 @@ int sbwt_index_from_rows(const uint64_t* const rows[4], const uint64_t* group, u
     }
   }
   ix->has_groups = group ? 1u : 0u;
-  int st = finish_index(ix, nullptr);
+  int st = finish_index(ix, nullptr, /*verify_groups=*/true);
   if (st != SBWT_OK) {
     destroy_index(ix);
     return st;

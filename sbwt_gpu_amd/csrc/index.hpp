@@ -46,14 +46,30 @@ DevIndex dev_view(const sbwt_index* ix);
 
 // Builds d_lines for ix->layout from d_rows and sets C from the device.
 int build_layout(sbwt_index* ix, hipStream_t s);
-// Finishes an index whose d_rows are filled: layout, C check, presearch.
-int finish_index(sbwt_index* ix, hipStream_t s);
+// Finishes an index whose d_rows are filled: layout, C check, suffix
+// groups, presearch. A missing group row is derived; a given one is checked
+// against the edges (cheap, for rows this library wrote) or, with
+// verify_groups, compared with the derived row word for word.
+int finish_index(sbwt_index* ix, hipStream_t s, bool verify_groups = false);
 // Frees everything (safe on partially built indexes).
 void destroy_index(sbwt_index* ix);
 // A fresh index shell with cfg applied (rows not allocated).
 sbwt_index* new_index(const sbwt_config* cfg, u32 k, u64 n);
 // Validates n and k against what the device layout supports.
 int check_shape(u64 n, u32 k);
+// Source column of every node's incoming edge (labels.hip): u32[n],
+// allocated here, freed by the caller.
+int build_pred(const sbwt_index* ix, u32** pred, hipStream_t s);
+// Writes the suffix-group-start row derived from the four rows to dst
+// (nwords words; labels.hip); the rank layout must be built.
+int derive_groups(const sbwt_index* ix, u64* dst, hipStream_t s);
+// SBWT_ERR_FORMAT unless the index's group row equals the derived one.
+int verify_groups(const sbwt_index* ix, hipStream_t s);
+// SBWT_ERR_FORMAT unless every column with an edge, and column 0, starts a
+// group of the index's group row (labels.hip).
+int check_groups(const sbwt_index* ix, hipStream_t s);
+// Labels of all nodes as top-aligned k-mers, ~0 / 0 for padded nodes.
+int node_labels(sbwt_index* ix, u64* d_labels, u8* d_full, hipStream_t s);
 // The persistent read search (stream.hip).
 int search_reads_persistent(const sbwt_index* ix, const u64* packed, const u64* nmask, const u64* rs,
                             const u64* off, const u64* work, const u32* items, u64 rows, u64 bound, u32 seg,

@@ -7,12 +7,9 @@
 // samples (each chunk's word 0 is its prefix count), and the bit inside the
 // chunk by popcounts.
 //
-// Enumeration walks each node's label backwards: the last character c of
-// node j is the c with C[c] <= j < C[c+1], and j's only incoming edge comes
-// from column select_c(j - C[c]) (the (j-C[c])-th c-edge), whose label ends
-// with the first k-1 characters of j's label. After k steps the whole label
-// is known; a walk that reaches the root (column 0) earlier belongs to a
-// '$'-padded node, which is not a k-mer of the index.
+// Enumeration takes every node's label (labels.hip: a backward walk over
+// the predecessor array), keeps the nodes with k real characters and
+// compacts them in node order, in calls of at most 2^28 items.
 #include <rocprim/device/device_select.hpp>
 
 #include "index.hpp"
@@ -83,28 +80,6 @@ __global__ void k_select(DevIndex ix, const u32* __restrict__ samples, u64 per_r
   out[q] = t < ones ? static_cast<u64>(select_dev<CB>(ix, samples, per_row, nchunks, c, static_cast<u32>(t))) : kNone;
 }
 
-// Label of node j as a top-aligned k-mer, or ~0 for a padded node.
-template <int CB>
-__global__ void k_labels(DevIndex ix, const u32* __restrict__ samples, u64 per_row, u64 nchunks,
-                         u64* __restrict__ out, u8* __restrict__ full) {
-  const u64 j = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x;
-  if (j >= ix.n) return;
-  u32 cur = static_cast<u32>(j);
-  u64 code = 0;
-  bool ok = true;
-  for (u32 step = 0; step < ix.k; step++) {
-    if (cur == 0) {
-      ok = false;
-      break;
-    }
-    const u32 c = cur >= ix.C[3] ? 3u : (cur >= ix.C[2] ? 2u : (cur >= ix.C[1] ? 1u : 0u));
-    code |= static_cast<u64>(c) << (62 - 2 * (ix.k - 1 - step));
-    cur = select_dev<CB>(ix, samples, per_row, nchunks, c, cur - c_of(ix, c));
-  }
-  out[j] = ok ? code : kNone;
-  full[j] = ok ? 1 : 0;
-}
-
 static int ensure_samples(sbwt_index* ix, hipStream_t s) {
   if (ix->d_select && ix->select_layout == ix->layout) return SBWT_OK;
   if (ix->d_select) {
@@ -157,46 +132,49 @@ int sbwt_select(sbwt_index* ix, uint32_t c, const uint64_t* d_ranks, uint64_t co
 int sbwt_node_labels(sbwt_index* ix, uint64_t* d_labels, uint8_t* d_full, void* stream) {
   clear_error();
   SBWT_REQUIRE(ix && d_labels && d_full, "arguments must not be NULL");
-  hipStream_t s = as_stream(stream);
-  SBWT_TRY(ensure_samples(ix, s));
-  const u64 nchunks = chunks_for(ix->n, cols_for_layout(ix->layout));
-  DevIndex v = dev_view(ix);
-  const u64 grid = (ix->n + 255) / 256;
-  switch (ix->layout) {
-    case 16: k_labels<16><<<grid, 256, 0, s>>>(v, ix->d_select, ix->select_per_row, nchunks, d_labels, d_full); break;
-    case 32: k_labels<32><<<grid, 256, 0, s>>>(v, ix->d_select, ix->select_per_row, nchunks, d_labels, d_full); break;
-    default: k_labels<64><<<grid, 256, 0, s>>>(v, ix->d_select, ix->select_per_row, nchunks, d_labels, d_full); break;
-  }
-  SBWT_HIP(hipGetLastError());
-  return SBWT_OK;
+  return node_labels(ix, d_labels, d_full, as_stream(stream));
 }
 
 int sbwt_enumerate_kmers(sbwt_index* ix, uint64_t* d_kmers, uint64_t* count_out, void* stream) {
   clear_error();
   SBWT_REQUIRE(ix && d_kmers && count_out, "arguments must not be NULL");
   hipStream_t s = as_stream(stream);
+  constexpr u64 kPiece = 1ull << 28;
   u64* labels = nullptr;
   u8* full = nullptr;
   void* tmp = nullptr;
   u64* d_cnt = nullptr;
-  size_t tb = 0;
   int st = SBWT_OK;
+  u64 done = 0;
   do {
     if (hipMalloc(&labels, ix->n * 8) != hipSuccess || hipMalloc(&full, ix->n) != hipSuccess ||
         hipMalloc(&d_cnt, 8) != hipSuccess) {
       st = fail(SBWT_ERR_HIP, "out of device memory for %llu labels", (unsigned long long)ix->n);
       break;
     }
-    if ((st = sbwt_node_labels(ix, labels, full, stream)) != SBWT_OK) break;
-    if (rocprim::select(nullptr, tb, labels, full, d_kmers, d_cnt, ix->n, s) != hipSuccess ||
-        hipMalloc(&tmp, tb ? tb : 1) != hipSuccess ||
-        rocprim::select(tmp, tb, labels, full, d_kmers, d_cnt, ix->n, s) != hipSuccess ||
-        hipMemcpyAsync(count_out, d_cnt, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess) {
-      st = fail(SBWT_ERR_HIP, "k-mer compaction failed");
+    if ((st = node_labels(ix, labels, full, s)) != SBWT_OK) break;
+    size_t tb = 0;
+    const u64 first = ix->n < kPiece ? ix->n : kPiece;
+    if (rocprim::select(nullptr, tb, labels, full, d_kmers, d_cnt, first, s) != hipSuccess ||
+        hipMalloc(&tmp, tb ? tb : 1) != hipSuccess) {
+      st = fail(SBWT_ERR_HIP, "k-mer compaction setup failed");
       break;
     }
+    for (u64 off = 0; off < ix->n && st == SBWT_OK; off += kPiece) {
+      const u64 len = ix->n - off < kPiece ? ix->n - off : kPiece;
+      size_t need = 0;
+      u64 got = 0;
+      if (rocprim::select(nullptr, need, labels + off, full + off, d_kmers + done, d_cnt, len, s) != hipSuccess ||
+          need > tb ||
+          rocprim::select(tmp, need, labels + off, full + off, d_kmers + done, d_cnt, len, s) != hipSuccess ||
+          hipMemcpyAsync(&got, d_cnt, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess)
+        st = fail(SBWT_ERR_HIP, "k-mer compaction failed");
+      done += got;
+    }
   } while (false);
+  if (st == SBWT_OK) *count_out = done;
+  (void)hipStreamSynchronize(s);
   if (labels) (void)hipFree(labels);
   if (full) (void)hipFree(full);
   if (tmp) (void)hipFree(tmp);

@@ -378,20 +378,119 @@ def test_uniform_kernel_refuses_batches_that_may_hold_a_4g_row():
     assert b"2^32" in L.sbwt_last_error()
 
 
-def test_index_without_group_row():
-    """Without suffix-group starts every k-mer is searched (certificates still
-    apply)."""
-    rng = np.random.default_rng(2)
-    g = genome(rng, 5000)
-    s = O.SBWT([g], 15)
-    ix = sg.Index.from_rows(s.rows, s.n, 15, config=sg.Config(presearch_depth=6))
-    reads = [g[:200], g[1000:1100] + "ACGTTT" + g[2000:2100], "".join(ACGT[rng.integers(0, 4, 300)])]
-    batch = sg.ReadBatch.from_host(reads, 15)
+GROUP_CASES = [
+    (["ACGTACGTTGCA"], 3), (["AAAAAAA"], 3), (["ACGTNACGTTTACG", "acgtacgt"], 3), (["TTTT", "AC"], 5),
+    (["CGTAC" * 4], 5), (["ACGT" * 10, "", "GGGGCCCC"], 1), (["ACGTTGCA" * 9], 32),
+]
+
+
+def _group_case(name, rng):
+    """Inputs whose group rows hold long groups: segmental copies, many
+    short sequences ('$'-padded nodes with shared suffixes), near-repeats."""
+    if name == "segmental":
+        g = genome(rng, 40_000)
+        for _ in range(30):
+            a, b, ln = int(rng.integers(0, 39_000)), int(rng.integers(0, 39_000)), int(rng.integers(50, 900))
+            ln = min(ln, 40_000 - max(a, b))
+            g = g[:b] + g[a:a + ln] + g[b + ln:]
+        return [g], 31
+    if name == "short_reads":
+        base = genome(rng, 2000)
+        return [base[p:p + int(rng.integers(5, 60))] for p in rng.integers(0, 1900, 800)], 21
+    if name == "near_repeats":
+        unit = genome(rng, 40)
+        return ["".join(unit[:i] + "ACGT"[int(rng.integers(0, 4))] + unit[i + 1:] for i in range(0, 40, 3)) * 3], 15
+    raise KeyError(name)
+
+
+@pytest.mark.parametrize("seqs,k", GROUP_CASES)
+def test_group_row_derived_tiny(seqs, k):
+    """Without a group row the index derives it on the device: equal to the
+    oracle builder's group row, and the read search streams."""
+    s = O.SBWT(seqs, k)
+    ix = sg.Index.from_rows(s.rows, s.n, k, config=sg.Config(presearch_depth=min(k, 3)))
+    assert ix.info["has_groups"]
+    np.testing.assert_array_equal(ix.rows()[4][: s.nwords], s.group)
+
+
+@pytest.mark.parametrize("case", ["segmental", "short_reads", "near_repeats"])
+@pytest.mark.parametrize("layout", [16, 64])
+def test_group_row_derived_matches_oracle_and_answers(case, layout):
+    rng = np.random.default_rng({"segmental": 81, "short_reads": 82, "near_repeats": 83}[case])
+    seqs, k = _group_case(case, rng)
+    s = O.SBWT(seqs, k)
+    ix = sg.Index.from_rows(s.rows, s.n, k, config=sg.Config(presearch_depth=8, layout=layout))
+    assert ix.info["has_groups"]
+    np.testing.assert_array_equal(ix.rows()[4][: s.nwords], s.group)
+    src = seqs[0] if len(seqs[0]) > 500 else "".join(seqs)
+    reads = _reads(rng, src, 300, err=0.02, nrate=0.01, lens=(k, 200))
     chars, rs = O.parse_lines("\n".join(reads).encode())
-    oix = oracle_index(s, 6)
-    want = O.search_reads(oix, chars, rs, 15)
+    want = O.search_reads(oracle_index(s, 8), chars, rs, k)
+    batch = sg.ReadBatch.from_host(reads, k)
     for kw in READ_VARIANTS:
-        np.testing.assert_array_equal(sg.to_u64(ix.search_reads(batch, **kw)), want)
+        np.testing.assert_array_equal(sg.to_u64(ix.search_reads(batch, **kw)), want, err_msg=str(kw))
+
+
+def test_four_reference_files_give_the_streaming_index(tmp_path):
+    """The reference's input is four bit-vector files and nothing else: the
+    loaded index has the group row (equal to the oracle's) and its read
+    search runs as fast as the built index's, streaming."""
+    rng = np.random.default_rng(77)
+    g = genome(rng, 200_000, (0.3, 0.2, 0.2, 0.3))
+    k = 31
+    s = O.SBWT([g], k)
+    paths = [str(tmp_path / f"{c}.bin") for c in "ACGT"]
+    for c in range(4):
+        O.bv_write(paths[c], s.rows[c], s.n)
+    ix = sg.Index.from_files(paths, k, config=sg.Config(presearch_depth=10))
+    assert ix.info["has_groups"]
+    np.testing.assert_array_equal(ix.rows()[4][: s.nwords], s.group)
+    reads = _reads(rng, g, 2000, err=0.01, lens=(150, 150))
+    batch = sg.ReadBatch.from_host(reads, k)
+    chars, rs = O.parse_lines("\n".join(reads).encode())
+    want = O.search_reads(oracle_index(s, 10), chars, rs, k)
+    np.testing.assert_array_equal(sg.to_u64(ix.search_reads(batch)), want)
+    # streaming: far fewer full searches than k-mers
+    ix.counters(reset=True)
+    ix.search_reads(batch, count=True)
+    ctr = ix.counters(reset=True)
+    assert ctr["kmers"] == batch.total_kmers and ctr["full_searches"] < 0.2 * ctr["kmers"]
+
+
+def test_group_row_that_does_not_belong_is_refused(tmp_path):
+    rng = np.random.default_rng(78)
+    g = genome(rng, 20_000)
+    s = O.SBWT([g], 17)
+    bad = s.group.copy()
+    # a column with an outgoing edge that is not marked as a group start
+    edges = s.rows[0] | s.rows[1] | s.rows[2] | s.rows[3]
+    w = int(np.flatnonzero(edges[1:])[0]) + 1
+    bit = 63 - int(np.flatnonzero(np.unpackbits(np.array([edges[w]]).view(np.uint8)[::-1]))[0])
+    bad[w] &= ~(np.uint64(1) << np.uint64(bit))
+    with pytest.raises(sg.SbwtError, match="does not belong"):
+        sg.Index.from_rows(s.rows, s.n, 17, group=bad)
+    # a group file that passes the edge check but is not this index's row:
+    # one group start without edges (the last k-mer of a short sequence)
+    # dropped
+    seqs, k = _group_case("short_reads", rng)
+    r = O.SBWT(seqs, k)
+    edges = r.rows[0] | r.rows[1] | r.rows[2] | r.rows[3]
+    bits = lambda a: np.unpackbits(a.astype(">u8").view(np.uint8))[: r.n].astype(bool)
+    cand = np.flatnonzero(bits(r.group) & ~bits(edges))
+    cand = cand[cand > 0]
+    assert cand.size, "no group start without edges"
+    j = int(cand[0])
+    og = r.group.copy()
+    og[j >> 6] &= ~(np.uint64(1) << np.uint64(63 - (j & 63)))
+    paths = [str(tmp_path / f"{c}.bin") for c in "ACGT"]
+    for c in range(4):
+        O.bv_write(paths[c], r.rows[c], r.n)
+    gfile = str(tmp_path / "grp.bin")
+    O.bv_write(gfile, og, r.n)
+    with pytest.raises(sg.SbwtError, match="does not belong"):
+        sg.Index.from_files(paths, k, group_path=gfile)
+    O.bv_write(gfile, r.group, r.n)
+    assert sg.Index.from_files(paths, k, group_path=gfile).info["has_groups"]
 
 
 @pytest.mark.parametrize("k,m", [(31, 12), (31, 0), (20, 8), (32, 10), (9, 4)])
