@@ -1,22 +1,30 @@
 #!/usr/bin/env python3
-"""Benchmark: k-mer membership queries/s of the SBWT read search on one MI355X.
+"""Benchmark: k-mer membership queries/s of the SBWT search on one MI355X.
 
-Workload (the plain GPU search at scale): a seeded 100 Mbp synthetic genome
-(GC 40%, 10% of it segmental copies of 1-50 kbp source segments at 1%
-substitutions), k = 31, presearch depth 12; 1,000,000 reads of 150 bp drawn
-from it with 1% substitutions, queried as all consecutive 31-mers
-(120,000,000 k-mers). The index is built on the GPU before timing; the reads
-are resident in HBM as ASCII when the timed region starts. One step = the
-whole query path over the batch: 2-bit packing + non-ACGT mask, per-read
-k-mer offsets, work prefix, and the read search kernel writing one node id
-(or ~0) per k-mer.
+Default workload (`--workload large`): two large indexes on one GPU, each
+generated and built on the GPU from fixed seeds and queried with 1,000,000,000
+independent 31-mers (50% sampled from the indexed sequences, 50% random)
+through the reference's per-k-mer search semantics:
+  * a 3 Gbp uniform-random genome (~3e9 distinct 31-mers, n ~ 3.0e9 columns);
+  * 20 haplotypes of a 150 Mbp uniform base genome at 0.1% SNP and 0.01%
+    indels of 1-10 bp (~0.25e9 columns).
+Each index runs three paths side by side on the same resident queries:
+plain, the learned prefilter (an MLP trained on the GPU plus a Bloom filter
+over its false negatives) before the masked search, and a Bloom filter over
+every indexed k-mer before the masked search; every path's answers must
+equal the plain ones bit for bit. The plain path is also timed with
+presearch tables of depth 13 and 12 beside the configured 17. `value` is the
+plain path on the 3 Gbp index. The 100 Mbp read batch (1M reads of 150 bp,
+~120M consecutive 31-mers, `--workload reads`) rides in the same line.
 
 Prints one JSON line. `roofline` prices the search kernel's algorithmic
 bytes (128 B per rank line touched, counted by the kernel's counted variant
-on the same batch) against the 8 TB/s HBM peak; `cpu_baseline` times the
-CPU oracle (a restatement of the reference's search: poppy rank, SB=1024,
-same presearch depth) on a bounded sample of the same k-mers, and checks
-that the GPU answers on that sample are bit-identical.
+on the same queries) against the 8 TB/s HBM peak and a measured
+random-line roof; `traffic` is FETCH_SIZE + WRITE_SIZE per launch from
+rocprofv3 --pmc child runs; `cpu_baseline` times the CPU oracle (a
+restatement of the reference's search: poppy rank, SB=1024) on a bounded
+sample of the same k-mers and checks that the GPU answers there are
+bit-identical.
 """
 from __future__ import annotations
 
@@ -67,15 +75,24 @@ def main() -> int:
     ap.add_argument("--no-roof", action="store_true", help="skip the random-gather roof measurement")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 3 times (HBM counter calibration) and exit")
-    ap.add_argument("--workload", choices=["reads", "large", "batched", "prefilter", "pangenome", "text"],
-                    default="reads",
-                    help="reads: the 100 Mbp read search (default); large: a 3 Gbp uniform-random index "
-                         "queried with independent 31-mers, 50%% present, through the reference's search "
-                         "semantics (the HBM-bound case); batched: 10,000 uneven read sets with 5%% N, "
-                         "autotuned, in HIP-graph-captured batches; prefilter: the learned negative "
-                         "prefilter (trained on the GPU) on a 90%%-negative read mix, filtered vs plain; "
-                         "pangenome: 20 haplotypes of a 150 Mbp base (0.1%% SNP, 0.01%% indel), 1B independent "
-                         "31-mers 50%% present, plain and prefilter side by side")
+    ap.add_argument("--workload", choices=["large", "uniform", "pangenome", "reads", "batched", "prefilter", "text"],
+                    default="large",
+                    help="large (default): the 3 Gbp uniform-random index and the 20-haplotype pangenome index, "
+                         "1B independent 31-mers each, 50%% present, plain / learned-filtered / Bloom-filtered "
+                         "side by side, plus the 100 Mbp read batch as an extra object; uniform or pangenome: "
+                         "one of the two large indexes alone; reads: the 100 Mbp read search (1M x 150 bp); "
+                         "batched: 10,000 uneven read sets with 5%% N, autotuned, in HIP-graph-captured "
+                         "batches; prefilter: the learned negative prefilter (trained on the GPU) on a "
+                         "90%%-negative read mix, filtered vs plain; text: query text to result text")
+    ap.add_argument("--no-reads", action="store_true", help="large: leave out the 100 Mbp read batch")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="large indexes: the plain path only (no prefilter training or filtered paths), e.g. for a "
+                         "kernel trace whose search dispatches are all timed ones")
+    ap.add_argument("--shallow-depths", default="13,12",
+                    help="large indexes: presearch depths also timed on the plain path (comma list; empty: none)")
+    ap.add_argument("--negatives-like", choices=["uniform", "index"], default="index",
+                    help="prefilter training negatives: uniform random k-mers, or random k-mers at the "
+                         "index's base composition (each confirmed absent by the plain search)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc child runs that measure HBM traffic per launch")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
@@ -98,7 +115,7 @@ def main() -> int:
                     help="large: also time an index image save and load (to a temporary directory)")
     args = ap.parse_args()
     if args.presearch is None:
-        args.presearch = 17 if args.workload in ("large", "pangenome") else 12
+        args.presearch = 17 if args.workload in ("large", "uniform", "pangenome") else 12
 
     if args.calib_copy:
         return calib_copy()
@@ -108,15 +125,17 @@ def main() -> int:
     if args.pmc_child:
         calib_copy()
         args.steps, args.warmup, args.no_cpu, args.no_roof = 1, 1, True, True
-    elif not args.no_traffic and args.workload in ("reads", "large") and args.gpus == 1:
+    elif (not args.no_traffic and args.workload in ("reads", "large", "uniform") and args.gpus == 1
+          and int(os.environ.get("WORLD_SIZE", "1")) == 1):
         # before this process touches the GPU: the counter runs are children
         traffic = measure_traffic(args)
     args.traffic = traffic
-    if args.workload in ("large", "batched", "prefilter", "pangenome", "text"):
-        if args.gpus != 1:
-            print("bench.py measures one GPU.", file=sys.stderr)
+    if args.workload != "reads":
+        if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+            print("bench.py measures one GPU: the index is replicated per GPU and queries shard by read set "
+                  "with no exchange, so a multi-GPU figure is not produced by this build.", file=sys.stderr)
             return 2
-        return {"large": run_large, "batched": run_batched, "prefilter": run_prefilter,
+        return {"large": run_large, "uniform": run_uniform, "batched": run_batched, "prefilter": run_prefilter,
                 "pangenome": run_pangenome, "text": run_text}[args.workload](args)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -124,6 +143,12 @@ def main() -> int:
               "with no exchange, so a multi-GPU figure is not produced by this build.", file=sys.stderr)
         return 2
 
+    print(json.dumps(run_reads(args)), flush=True)
+    return 0
+
+
+def run_reads(args) -> dict:
+    """The 100 Mbp read batch (the body of the `reads` workload); returns its line."""
     import torch
 
     import sbwt_gpu_amd as sg
@@ -262,117 +287,348 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
-    return 0
+    return line
 
 
-def run_large(args) -> int:
-    """A 3 Gbp uniform-random genome (generated on the GPU from a fixed seed),
-    its SBWT built on the GPU (~3e9 columns, HBM-resident: far beyond the
-    256 MiB last-level cache), and `queries` 31-mers, even ones drawn from
-    uniform genome positions (present) and odd ones from a second random
-    sequence (absent w.h.p.), each searched independently with the
-    reference's semantics (presearch gather + up to k-m narrowing steps)."""
+
+def _time_path(run, args):
+    """W untimed runs, then K timed runs between synchronisations; run(ev)
+    records ev[0] (start), ev[1] (filter done) and ev[2] (search done) on
+    the current stream. Returns (seconds per step, filter ms, search ms)."""
+    import torch
+
+    for _ in range(args.warmup):
+        run(None)
+    torch.cuda.synchronize()
+    nsteps = max(args.steps, 1)
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(nsteps)]
+    t0 = time.perf_counter()
+    for i in range(nsteps):
+        run(evs[i])
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / nsteps
+    filt = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
+    search = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+    return el, filt, search
+
+
+def large_index(kind: str, args) -> dict:
+    """One large index on one GPU, generated and built on the GPU from fixed
+    seeds, queried with `queries` independent 31-mers, 50/50 present and
+    absent, through the reference's per-k-mer search semantics (presearch
+    gather + up to k - m narrowing steps, include/gpu/search_kernel.hpp:61-104):
+      uniform:   a 3 Gbp uniform-random genome (~3e9 distinct 31-mers); even
+                 queries at uniform genome positions, odd ones from a second
+                 random sequence;
+      pangenome: `haplotypes` copies of a 150 Mbp uniform base genome, each
+                 with 0.1% SNPs and 0.01% indels of 1-10 bp; even queries at
+                 uniform positions of uniformly chosen haplotypes, odd ones
+                 uniform random k-mers.
+    Three paths run side by side on the same resident queries: plain; the
+    learned prefilter (MLP trained on the GPU + Bloom filter over its false
+    negatives) before the masked search; and a Bloom filter over every
+    indexed k-mer before the masked search. Every path's answers must equal
+    the plain ones bit for bit, and every sampled (even) query must be found.
+    Beside the configured depth the plain path is also timed with presearch
+    tables of depth 13 and 12 (the reference's clamp is 5..13,
+    include/parameters.hpp:255-259)."""
+    import torch
+
+    import sbwt_gpu_amd as sg
+    from sbwt_gpu_amd import _capi, synth
+    from sbwt_gpu_amd.api import _ptr, _stream
+    from sbwt_gpu_amd.learned_filter import random_kmers, train_prefilter
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    k, Q = args.k, args.queries
+    L = _capi.load()
+    check = _capi.check
+    s = _stream()
+    gen = torch.Generator(device=dev)
+    t = time.time()
+    if kind == "uniform":
+        G, R = args.large_bp, 64_000_000  # R: the absent-query source
+        gen.manual_seed(8)
+        chars = torch.randint(0, 4, (G + R,), dtype=torch.uint8, device=dev, generator=gen)
+        # 2-bit code -> ASCII A, C, G, T
+        chars.mul_(2).add_(65)
+        chars.add_((chars >= 69).to(torch.uint8) * 2)
+        chars.add_((chars == 73).to(torch.uint8) * 11)
+        starts = torch.tensor([0, G], dtype=torch.int64, device=dev)
+        desc = f"{G / 1e9:g} Gbp uniform-random genome"
+    else:
+        gen.manual_seed(31)
+        base = torch.randint(0, 4, (args.base_bp,), dtype=torch.uint8, device=dev, generator=gen)
+        chars, starts = synth.gpu_haplotypes(base, args.haplotypes, seed=32)
+        del base
+        G, R = chars.numel(), 0
+        desc = (f"pangenome: {args.haplotypes} haplotypes of a {args.base_bp / 1e6:g} Mbp uniform base genome "
+                f"(0.1% SNP, 0.01% indel 1-10 bp), {G / 1e9:.3f} Gbp")
+    torch.cuda.synchronize()
+    gen_s = time.time() - t
+    log(f"{kind}: {G + R} bases generated on the GPU in {gen_s:.1f}s")
+    t = time.time()
+    ix = sg.Index.build(chars[:G], k, sg.Config(presearch_depth=args.presearch, layout=args.layout), starts=starts)
+    info = ix.info
+    build_s = time.time() - t
+    log(f"{kind}: index built on GPU in {build_s:.1f}s: n={info['n']} columns, "
+        f"{info['device_bytes'] / 2**30:.2f} GiB, layout L{info['layout']}, m={info['presearch_depth']}")
+    packed = torch.empty(L.sbwt_packed_words(G + R), dtype=torch.int64, device=dev)
+    check(L.sbwt_pack(_ptr(chars), G + R, _ptr(packed), None, s))
+    del chars
+    half = (Q + 1) // 2
+    if kind == "uniform":
+        pos = torch.empty(Q, dtype=torch.int64, device=dev)
+        pos[0::2] = torch.randint(0, G - k + 1, (half,), device=dev, generator=gen)
+        pos[1::2] = G + torch.randint(0, R - k + 1, (Q // 2,), device=dev, generator=gen)
+        kmers = torch.empty_like(pos)
+        check(L.sbwt_extract_kmers(_ptr(packed), _ptr(pos), Q, _ptr(kmers), s))
+        del pos
+    else:
+        lens = torch.diff(starts)
+        hap = torch.randint(0, lens.numel(), (half,), device=dev, generator=gen)
+        off = (torch.rand(half, device=dev, generator=gen, dtype=torch.float64) * (lens - k + 1)[hap]).to(torch.int64)
+        pos = starts[:-1][hap] + off
+        del hap, off
+        kpos = torch.empty(half, dtype=torch.int64, device=dev)
+        check(L.sbwt_extract_kmers(_ptr(packed), _ptr(pos), half, _ptr(kpos), s))
+        kmers = torch.empty(Q, dtype=torch.int64, device=dev)
+        kmers[0::2] = kpos
+        kmers[1::2] = random_kmers(Q // 2, k, gen)
+        del pos, kpos
+    del packed, starts
+    out = torch.empty_like(kmers)
+    maybe = torch.empty(Q, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    log(f"{kind}: {Q} queries ready")
+    stream = torch.cuda.current_stream()
+
+    def plain(ev):
+        if ev:
+            ev[0].record(stream)
+            ev[1].record(stream)
+        check(L.sbwt_search_kmers(ix._h, _ptr(kmers), Q, _ptr(out), s))
+        if ev:
+            ev[2].record(stream)
+
+    el, _, kern_ms = _time_path(plain, args)
+    if args.pmc_child:  # the counter runs need the timed launches only
+        return {}
+    ref = out.clone()
+    missing = int((ref[0::2] == -1).sum().item())
+    found = int((ref != -1).sum().item())
+    log(f"{kind} plain: {Q / el / 1e9:.3f} G k-mers/s ({el * 1e3:.2f} ms/step, kernel {kern_ms:.2f} ms), "
+        f"found {found}/{Q}, sampled k-mers missing: {missing}")
+    # algorithmic bytes: the counted variant on the same queries (untimed)
+    ix.counters(reset=True)
+    ix.search_kmers(kmers, out=out, count=True)
+    torch.cuda.synchronize()
+    ctr = ix.counters(reset=True)
+    counted_same = bool(torch.equal(out, ref))
+    lines = int(ctr["lines"])
+    kern_s = kern_ms / 1e3
+    achieved = lines * 128 / kern_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None, "kernel": "k_search_kmers",
+                "kernel_avg_ms": round(kern_ms, 3), "lines_per_launch": lines,
+                "bytes_per_kmer": round(lines * 128 / Q, 2), "steps_per_launch": int(ctr["steps"]),
+                "floor_bytes_per_kmer": round(128 * int(ctr["steps"]) / Q, 2),
+                "index_bytes": info["device_bytes"], "counted_equals_timed": counted_same}
+    res = {"workload": desc, "bases": G, "index_columns": info["n"], "index_build_s": round(build_s, 1),
+           "generate_s": round(gen_s, 1), "layout": f"L{info['layout']}", "presearch": info["presearch_depth"],
+           "queries": Q, "found_fraction": round(found / Q, 4), "sampled_kmers_all_found": missing == 0,
+           "roofline": roofline}
+
+    paths = {"plain": {"kmers_per_s": round(Q / el, 1), "ms_per_step": round(el * 1e3, 3),
+                       "filter_kernel_ms": 0.0, "search_kernel_ms": round(kern_ms, 3), "maybe_fraction": None,
+                       "answers_equal_plain": True}}
+    res["paths"] = paths
+    if not args.plain_only:
+        _filtered_paths(kind, ix, kmers, out, ref, maybe, paths, res, args)
+    torch.cuda.empty_cache()
+
+    # ---- the CPU baseline on a sample, with GPU parity ----------------------
+    if not args.no_cpu:
+        res_h = ref[: 50_000_000].cpu().numpy().view(np.uint64)
+        res["cpu_baseline"] = cpu_baseline_kmers(ix, kmers, res_h, k, args)
+
+    # ---- shallower presearch tables (untimed rebuild, then timed steps) ------
+    depths = {}
+    for m in [int(x) for x in args.shallow_depths.split(",") if x]:
+        if m >= args.presearch:
+            continue
+        t = time.time()
+        ix.presearch(m)
+        torch.cuda.synchronize()
+        build_m = time.time() - t
+        sub = argparse.Namespace(**{**vars(args), "steps": max(3, min(args.steps, 5)), "warmup": 1})
+        e, _, km = _time_path(plain, sub)
+        same = bool(torch.equal(out, ref))
+        ix.counters(reset=True)
+        ix.search_kmers(kmers, out=out, count=True)
+        torch.cuda.synchronize()
+        c = ix.counters(reset=True)
+        depths[str(m)] = {"kmers_per_s": round(Q / e, 1), "kernel_ms": round(km, 3),
+                          "lines_per_kmer": round(int(c["lines"]) / Q, 3),
+                          "achieved_GBps": round(int(c["lines"]) * 128 / (km / 1e3) / 1e9, 1),
+                          "table_bytes": 8 << (2 * m), "table_build_s": round(build_m, 2),
+                          "answers_equal_plain": same}
+        log(f"{kind} m={m}: {Q / e / 1e9:.3f} G k-mers/s, kernel {km:.2f} ms, "
+            f"{int(c['lines']) / Q:.2f} lines/k-mer; answers equal: {same}")
+    res["shallower_presearch"] = depths
+    if args.checkpoint and kind == "uniform":
+        ix.presearch(args.presearch)
+        res["checkpoint"] = image_round_trip(ix, kmers, out, ref.cpu().numpy().view(np.uint64))  # closes ix
+    ix.close()
+    del kmers, out, ref, maybe
+    torch.cuda.empty_cache()
+    return res
+
+
+def _filtered_paths(kind, ix, kmers, out, ref, maybe, paths, res, args):
+    """The learned prefilter and the Bloom filter over every indexed k-mer,
+    each before the masked search, timed on the same queries as the plain
+    path; their answers are compared with the plain ones (ref) on the device."""
     import torch
 
     import sbwt_gpu_amd as sg
     from sbwt_gpu_amd import _capi
     from sbwt_gpu_amd.api import _ptr, _stream
+    from sbwt_gpu_amd.learned_filter import train_prefilter
 
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
-    dev = torch.device("cuda:0")
-    k, G, Q = args.k, args.large_bp, args.queries
-    R = 64_000_000  # absent-query source
-    L = _capi.load()
-    s = _stream()
-    gen = torch.Generator(device=dev)
-    gen.manual_seed(8)
-    t = time.time()
-    chars = torch.randint(0, 4, (G + R,), dtype=torch.uint8, device=dev, generator=gen)
-    # 2-bit code -> ASCII A, C, G, T
-    chars.mul_(2).add_(65)
-    chars.add_((chars >= 69).to(torch.uint8) * 2)
-    chars.add_((chars == 73).to(torch.uint8) * 11)
-    torch.cuda.synchronize()
-    log(f"{G + R} random bases generated on the GPU in {time.time() - t:.1f}s")
-    t = time.time()
-    ix = sg.Index.build(chars[:G], k, sg.Config(presearch_depth=args.presearch, layout=args.layout),
-                        starts=torch.tensor([0, G], dtype=torch.int64, device=dev))
-    info = ix.info
-    build_s = time.time() - t
-    log(f"index built on GPU in {build_s:.1f}s: n={info['n']} columns, {info['device_bytes'] / 2**30:.2f} GiB, "
-        f"layout L{info['layout']}, m={info['presearch_depth']}")
-    packed = torch.empty(L.sbwt_packed_words(G + R), dtype=torch.int64, device=dev)
-    _capi.check(L.sbwt_pack(_ptr(chars), G + R, _ptr(packed), None, s))
-    del chars
-    pos = torch.empty(Q, dtype=torch.int64, device=dev)
-    pos[0::2] = torch.randint(0, G - k + 1, ((Q + 1) // 2,), device=dev, generator=gen)
-    pos[1::2] = G + torch.randint(0, R - k + 1, (Q // 2,), device=dev, generator=gen)
-    kmers = torch.empty_like(pos)
-    _capi.check(L.sbwt_extract_kmers(_ptr(packed), _ptr(pos), Q, _ptr(kmers), s))
-    del pos, packed
-    out = torch.empty_like(kmers)
-    torch.cuda.synchronize()
+    L, check, s = _capi.load(), _capi.check, _stream()
+    k, Q = args.k, kmers.numel()
     stream = torch.cuda.current_stream()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * args.steps)]
-    for _ in range(args.warmup):
-        _capi.check(L.sbwt_search_kmers(ix._h, _ptr(kmers), Q, _ptr(out), s))
-    torch.cuda.synchronize()
-    log(f"timing {args.steps} steps of {Q} k-mers")
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        ev[2 * i].record(stream)
-        _capi.check(L.sbwt_search_kmers(ix._h, _ptr(kmers), Q, _ptr(out), s))
-        ev[2 * i + 1].record(stream)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    kern_avg_s = float(np.mean([ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(args.steps)])) / 1e3
-    value = Q * args.steps / elapsed
-    res = out.cpu().numpy().view(np.uint64)
-    found = int((res != np.uint64(0xFFFFFFFFFFFFFFFF)).sum())
-    ix.counters(reset=True)
-    ix.search_kmers(kmers, out=out, count=True)
-    torch.cuda.synchronize()
-    ctr = ix.counters(reset=True)
-    lines = int(ctr["lines"])
-    achieved = lines * 128 / kern_avg_s / 1e9
-    log(f"{value / 1e9:.3f} G k-mers/s, kernel {kern_avg_s * 1e3:.2f} ms, {lines / Q:.2f} lines/k-mer, "
-        f"{achieved:.0f} GB/s of 128-B lines, found {found}/{Q}")
-    traffic = traffic_bytes(args.traffic)
-    if traffic is not None:  # the counter runs search 250M k-mers per launch; scale to this launch
-        traffic = round(traffic * Q / args.traffic["queries_per_launch"])
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
-                "traffic_detail": args.traffic, "kernel": "k_search_kmers",
-                "kernel_avg_ms": round(kern_avg_s * 1e3, 3), "lines_per_launch": lines,
-                "bytes_per_kmer": round(lines * 128 / Q, 2), "steps_per_launch": int(ctr["steps"]),
-                "floor_bytes_per_kmer": round(128 * int(ctr["steps"]) / Q, 2),
-                "index_bytes": info["device_bytes"]}
-    fetch = ((args.traffic or {}).get("counters") or {}).get("FETCH_SIZE")
+    t = time.time()
+    pf = train_prefilter(ix, positives=args.train_positives, negatives=args.train_negatives,
+                         epochs=args.train_epochs, batch=args.train_batch, lr=args.lr, seed=7,
+                         threshold=args.threshold, fpr=0.01, negatives_like=args.negatives_like, log=log)
+    train_s = time.time() - t
+    log(f"{kind}: prefilter ready in {train_s:.1f}s: {pf.stats}")
+    t = time.time()
+    allpos = ix.enumerate_kmers()
+    bloom_all = sg.Bloom(k, allpos.numel(), 0.01)
+    bloom_all.insert(allpos)
+    del allpos
+    torch.cuda.empty_cache()
+    bloom_s = time.time() - t
+
+    def filtered(kind_):
+        def run(ev):
+            if ev:
+                ev[0].record(stream)
+            if kind_ == "learned":
+                pf.maybe(kmers, out=maybe)
+            else:
+                check(L.sbwt_bloom_query(bloom_all._h, _ptr(kmers), Q, _ptr(maybe), s))
+            if ev:
+                ev[1].record(stream)
+            check(L.sbwt_search_kmers_masked(ix._h, _ptr(kmers), Q, _ptr(maybe), _ptr(out), 0, s))
+            if ev:
+                ev[2].record(stream)
+        return run
+
+    for name, run in (("filtered", filtered("learned")), ("bloom_only", filtered("bloom"))):
+        e, fm, sm = _time_path(run, args)
+        same = bool(torch.equal(out, ref))
+        mf = round(float(maybe.float().mean().item()), 4)
+        paths[name] = {"kmers_per_s": round(Q / e, 1), "ms_per_step": round(e * 1e3, 3),
+                       "filter_kernel_ms": round(fm, 3), "search_kernel_ms": round(sm, 3), "maybe_fraction": mf,
+                       "answers_equal_plain": same}
+        log(f"{kind} {name}: {Q / e / 1e9:.3f} G k-mers/s ({e * 1e3:.2f} ms/step; filter {fm:.2f} ms, "
+            f"search {sm:.2f} ms, maybe {mf}); answers equal plain: {same}")
+    res["filtered_vs_plain"] = round(paths["filtered"]["kmers_per_s"] / paths["plain"]["kmers_per_s"], 3)
+    res["bloom_only_vs_plain"] = round(paths["bloom_only"]["kmers_per_s"] / paths["plain"]["kmers_per_s"], 3)
+    res["prefilter"] = {"train": pf.stats, "train_total_s": round(train_s, 1),
+                        "learned_bloom_bytes": pf.bloom.bytes if pf.bloom else 0,
+                        "model_bytes": int(L.sbwt_mlp_param_count(pf.mlp._h)) * 2,
+                        "bloom_only_bytes": bloom_all.bytes, "bloom_only_build_s": round(bloom_s, 1)}
+    del pf, bloom_all
+    torch.cuda.empty_cache()
+
+
+def _traffic_into(roofline: dict, traffic: dict | None, queries: int) -> None:
+    """Adds the rocprofv3 counter runs (which search queries_per_launch
+    k-mers) to a large-index roofline, scaled to its launch."""
+    roofline["traffic_detail"] = traffic
+    t = traffic_bytes(traffic)
+    if t is not None:
+        roofline["traffic"] = round(t * queries / traffic["queries_per_launch"])
+    fetch = ((traffic or {}).get("counters") or {}).get("FETCH_SIZE")
     if fetch:
-        roofline["hbm_fetch_bytes_per_kmer"] = round(fetch["bytes"] / args.traffic["queries_per_launch"], 2)
-    if (args.traffic or {}).get("tcc"):
-        roofline["tcc_hit_rate"] = args.traffic["tcc"]["hit_rate"]
+        roofline["hbm_fetch_bytes_per_kmer"] = round(fetch["bytes"] / traffic["queries_per_launch"], 2)
+    if (traffic or {}).get("tcc"):
+        roofline["tcc_hit_rate"] = traffic["tcc"]["hit_rate"]
+
+
+def _large_line(res: dict, args) -> dict:
+    p = res.get("paths", {}).get("plain") or {}
+    return {"metric": METRIC, "value": p.get("kmers_per_s"), "unit": "k-mer queries/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": p.get("ms_per_step"),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+            "data": "synthetic (seeded, generated on the GPU)"}
+
+
+def run_uniform(args) -> int:
+    res = large_index("uniform", args)
+    if args.pmc_child:
+        return 0
+    _traffic_into(res["roofline"], args.traffic, args.queries)
     if not args.no_roof:
         roof = gather_roof()
-        roofline["measured_random_roof"] = roof
-        roofline["frac_of_measured_roof"] = round(achieved / roof["GBps_128B_lines"], 4)
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline_kmers(ix, kmers, res, k, args)
-    checkpoint = image_round_trip(ix, kmers, out, res) if args.checkpoint else None  # closes ix
-    line = {
-        "metric": METRIC, "value": round(value, 1), "unit": "k-mer queries/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic (seeded, generated on the GPU)",
-        "config": {"workload": f"large index: {G / 1e9:g} Gbp uniform-random genome, k={k}, presearch "
-                               f"{args.presearch}; {Q} independent 31-mers, 50% present",
-                   "index_columns": info["n"], "index_build_s": round(build_s, 1), "layout": f"L{info['layout']}",
-                   "found_fraction": round(found / Q, 4)},
-        "roofline": roofline, "cpu_baseline": cpu,
+        res["roofline"]["measured_random_roof"] = roof
+        res["roofline"]["frac_of_measured_roof"] = round(res["roofline"]["achieved"] / roof["GBps_128B_lines"], 4)
+    line = _large_line(res, args)
+    line["config"] = {k: v for k, v in res.items() if k not in ("roofline", "cpu_baseline")}
+    line["roofline"], line["cpu_baseline"] = res["roofline"], res.get("cpu_baseline")
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def run_pangenome(args) -> int:
+    res = large_index("pangenome", args)
+    line = _large_line(res, args)
+    line["config"] = {k: v for k, v in res.items() if k not in ("roofline", "cpu_baseline")}
+    line["roofline"], line["cpu_baseline"] = res["roofline"], res.get("cpu_baseline")
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def run_large(args) -> int:
+    """The large-index workload on one GPU: the 3 Gbp uniform-random index
+    and the 20-haplotype pangenome index, 1B queries each (50/50), plain and
+    filtered side by side (large_index). `value` is the plain path on the
+    3 Gbp index; the pangenome and the 100 Mbp read batch (run_reads) ride in
+    the same line as their own objects."""
+    import torch
+
+    uni = large_index("uniform", args)
+    _traffic_into(uni["roofline"], args.traffic, args.queries)
+    if not args.no_roof:
+        roof = gather_roof()
+        uni["roofline"]["measured_random_roof"] = roof
+        uni["roofline"]["frac_of_measured_roof"] = round(uni["roofline"]["achieved"] / roof["GBps_128B_lines"], 4)
+    torch.cuda.empty_cache()
+    pan = large_index("pangenome", args)
+    torch.cuda.empty_cache()
+    reads = None
+    if not args.no_reads:
+        rargs = argparse.Namespace(**{**vars(args), "presearch": 12, "traffic": None, "no_roof": True})
+        reads = run_reads(rargs)
+    line = _large_line(uni, args)
+    line["config"] = {
+        "workload": "large index, one GPU: a 3 Gbp uniform-random genome index and a 20-haplotype pangenome "
+                    f"index, {args.queries} independent 31-mers each, 50% present; plain, learned-filtered and "
+                    "Bloom-filtered searches side by side; value = plain on the 3 Gbp index",
+        "uniform_3Gbp": {k: v for k, v in uni.items() if k not in ("roofline", "cpu_baseline")},
+        "pangenome": {k: v for k, v in pan.items() if k != "cpu_baseline"},
+        "filtered_vs_plain": uni.get("filtered_vs_plain"),
     }
-    if checkpoint:
-        line["config"]["checkpoint"] = checkpoint
+    line["roofline"], line["cpu_baseline"] = uni["roofline"], uni.get("cpu_baseline")
+    line["pangenome_cpu_baseline"] = pan.get("cpu_baseline")
+    line["reads_100Mbp_1Mx150bp"] = reads
     print(json.dumps(line), flush=True)
     return 0
 
@@ -486,7 +742,7 @@ def run_batched(args) -> int:
 
     tune = None
     seg = args.seg
-    persistent = args.mode.startswith("persistent")
+    kflags = autotune.kernel_flags("persistent" if args.mode.startswith("persistent") else "lane")
     if not args.no_autotune:
         sample_chars, sample_rel, _, _ = batches[0]
         t = time.time()
@@ -494,13 +750,13 @@ def run_batched(args) -> int:
         # one batch, whose table entries stay cached) but lose over the pass
         tune = autotune.autotune(ix, sample_chars, sample_rel, segs=(64, 256, 1024), budget_s=90.0)
         seg = tune["best"]["segment_kmers"]
-        persistent = tune["best"]["kernel"] == "persistent"
+        kflags = autotune.kernel_flags(tune["best"]["kernel"])
         log(f"autotune ({time.time() - t:.1f}s, {len(tune['trials'])} trials): {tune['best']}")
     plan = sg.ReadsPlan(max_rows, max_chars, seg)
 
     def run_all(stream=None):
         for bc, brel, o0, o1 in batches:
-            plan.search(ix, bc, brel, out[o0:o1] if o1 > o0 else out[:1], stream=stream, persistent=persistent)
+            plan.search(ix, bc, brel, out[o0:o1] if o1 > o0 else out[:1], stream=stream, **kflags)
 
     run_all()
     torch.cuda.synchronize()
@@ -534,14 +790,15 @@ def run_batched(args) -> int:
     # algorithmic lines: the counted instantiation over the same batches (untimed)
     ix.counters(reset=True)
     for bc, brel, o0, o1 in batches:
-        plan.search(ix, bc, brel, out[o0:o1] if o1 > o0 else out[:1], persistent=persistent, count=True)
+        plan.search(ix, bc, brel, out[o0:o1] if o1 > o0 else out[:1], count=True, **kflags)
     torch.cuda.synchronize()
     lines = int(ix.counters(reset=True)["lines"])
     counted_same = bool(torch.equal(out, eager_ref))
     achieved = lines * 128 / graph_s / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-                "kernel": "k_reads_uniform" if persistent else "k_search_reads", "kernel_avg_ms": None,
+                "kernel": "k_reads_uniform" if kflags["persistent"] else "k_search_reads", "kernel_avg_ms": None,
+                "run_items": kflags["runs"],
                 "pass_ms": round(graph_s * 1e3, 3), "lines_per_pass": lines,
                 "bytes_per_kmer": round(lines * 128 / total, 2), "counted_equals_timed": counted_same,
                 "note": "over the whole graph pass (every batch's packing, offsets, work prefix and search): "
@@ -563,11 +820,23 @@ def run_batched(args) -> int:
                                f"{args.genome_bp / 1e6:g} Mbp index; HIP-graph batches",
                    "reads": rows, "bases": int(starts_h[-1]), "kmers_per_step": total, "batches": len(batches),
                    "eager_kmers_per_s": round(total / eager_s, 1), "graph_replays_equal_eager": replay_ok,
-                   "autotune_best": tune["best"] if tune else None, "found_fraction": round(found / total, 4)},
+                   "autotune_best": tune["best"] if tune else None,
+                   "autotune_best_per_kernel": _best_per_kernel(tune), "found_fraction": round(found / total, 4)},
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def _best_per_kernel(tune):
+    """The fastest autotune trial of each kernel (work items x kernel)."""
+    if not tune:
+        return None
+    best = {}
+    for t in tune["trials"]:
+        if t["kernel"] not in best or t["ms"] < best[t["kernel"]]["ms"]:
+            best[t["kernel"]] = t
+    return best
 
 
 def run_text(args) -> int:
@@ -936,165 +1205,6 @@ def run_prefilter(args) -> int:
     return 0
 
 
-def run_pangenome(args) -> int:
-    """Pangenome-style large index: `haplotypes` copies of a seeded
-    `base_bp` uniform genome, each with 0.1% substitutions and 0.01% indels
-    (1-10 bp, half insertions), generated and indexed on the GPU.
-    `queries` independent 31-mers: even ones from uniform positions of
-    uniformly chosen haplotypes (present), odd ones uniform random (absent
-    w.h.p.). Timed side by side on the same resident queries: the plain
-    search (the reference's per-k-mer semantics), the learned prefilter
-    (MLP trained on the GPU + Bloom filter over its false negatives) with
-    the masked search, and a Bloom filter over every indexed k-mer with the
-    masked search. Filtered answers must equal the plain ones bit for bit."""
-    import torch
-
-    import sbwt_gpu_amd as sg
-    from sbwt_gpu_amd import _capi, synth
-    from sbwt_gpu_amd.api import _ptr, _stream
-    from sbwt_gpu_amd.learned_filter import random_kmers, train_prefilter
-
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
-    dev = torch.device("cuda:0")
-    k, Q = args.k, args.queries
-    L = _capi.load()
-    s = _stream()
-    t = time.time()
-    gen = torch.Generator(device=dev)
-    gen.manual_seed(31)
-    base = torch.randint(0, 4, (args.base_bp,), dtype=torch.uint8, device=dev, generator=gen)
-    chars, starts = synth.gpu_haplotypes(base, args.haplotypes, seed=32)
-    del base
-    torch.cuda.synchronize()
-    G = chars.numel()
-    log(f"{args.haplotypes} haplotypes, {G} bases generated on the GPU in {time.time() - t:.1f}s")
-    t = time.time()
-    ix = sg.Index.build(chars, k, sg.Config(presearch_depth=args.presearch, layout=args.layout), starts=starts)
-    info = ix.info
-    torch.cuda.synchronize()
-    build_s = time.time() - t
-    log(f"index built on GPU in {build_s:.1f}s: n={info['n']} columns, {info['device_bytes'] / 2**30:.2f} GiB")
-    packed = torch.empty(L.sbwt_packed_words(G), dtype=torch.int64, device=dev)
-    _capi.check(L.sbwt_pack(_ptr(chars), G, _ptr(packed), None, s))
-    del chars
-    starts_h = starts.cpu().numpy()
-    lens = np.diff(starts_h)
-    nh = lens.size
-    half = (Q + 1) // 2
-    hap = torch.randint(0, nh, (half,), device=dev, generator=gen)
-    span = torch.from_numpy(lens - k + 1).to(dev)[hap]
-    off = (torch.rand(half, device=dev, generator=gen, dtype=torch.float64) * span).to(torch.int64)
-    pos = starts[:-1][hap] + off
-    del hap, span, off
-    kmers = torch.empty(Q, dtype=torch.int64, device=dev)
-    kpos = torch.empty(half, dtype=torch.int64, device=dev)
-    _capi.check(L.sbwt_extract_kmers(_ptr(packed), _ptr(pos), half, _ptr(kpos), s))
-    kmers[0::2] = kpos
-    del pos, kpos, packed
-    kmers[1::2] = random_kmers(Q // 2, k, gen)
-    out = torch.empty_like(kmers)
-    maybe = torch.empty(Q, dtype=torch.uint8, device=dev)
-    torch.cuda.synchronize()
-    log(f"{Q} queries ready")
-
-    t = time.time()
-    pf = train_prefilter(ix, positives=args.train_positives, negatives=args.train_negatives,
-                         epochs=args.train_epochs, batch=args.train_batch, lr=args.lr, seed=7,
-                         threshold=args.threshold, fpr=0.01, log=log)
-    train_total_s = time.time() - t
-    log(f"prefilter ready in {train_total_s:.1f}s: {pf.stats}")
-    allpos = ix.enumerate_kmers()
-    bloom_all = sg.Bloom(k, allpos.numel(), 0.01)
-    bloom_all.insert(allpos)
-    del allpos
-
-    stream = torch.cuda.current_stream()
-
-    def plain(ev=None):
-        if ev:
-            ev[1].record(stream)
-        _capi.check(L.sbwt_search_kmers(ix._h, _ptr(kmers), Q, _ptr(out), s))
-        if ev:
-            ev[2].record(stream)
-
-    def filtered(kind):
-        def run(ev=None):
-            if ev:
-                ev[0].record(stream)
-            if kind == "learned":
-                pf.maybe(kmers, out=maybe)
-            else:
-                _capi.check(L.sbwt_bloom_query(bloom_all._h, _ptr(kmers), Q, _ptr(maybe), s))
-            if ev:
-                ev[1].record(stream)
-            _capi.check(L.sbwt_search_kmers_masked(ix._h, _ptr(kmers), Q, _ptr(maybe), _ptr(out), 0, s))
-            if ev:
-                ev[2].record(stream)
-        return run
-
-    paths = {"plain": plain, "filtered": filtered("learned"), "bloom_only": filtered("bloom")}
-    results, answers = {}, {}
-    nsteps = max(args.steps, 1)
-    for name, run in paths.items():
-        for _ in range(args.warmup):
-            run()
-        torch.cuda.synchronize()
-        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(nsteps)]
-        t0 = time.perf_counter()
-        for i in range(nsteps):
-            run(evs[i])
-        torch.cuda.synchronize()
-        el = (time.perf_counter() - t0) / nsteps
-        filt_ms = 0.0 if name == "plain" else float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
-        search_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
-        answers[name] = out.cpu().numpy().view(np.uint64).copy()
-        mrate = None if name == "plain" else round(float(maybe.float().mean().item()), 4)
-        results[name] = {"kmers_per_s": round(Q / el, 1), "ms_per_step": round(el * 1e3, 3),
-                         "filter_kernel_ms": round(filt_ms, 3), "search_kernel_ms": round(search_ms, 3),
-                         "maybe_fraction": mrate}
-        log(f"{name}: {Q / el / 1e9:.3f} G k-mers/s ({el * 1e3:.2f} ms/step; filter {filt_ms:.2f} ms, "
-            f"search {search_ms:.2f} ms, maybe {mrate})")
-    res = answers["plain"]
-    exact = {n: bool(np.array_equal(a, res)) for n, a in answers.items()}
-    none = np.uint64(0xFFFFFFFFFFFFFFFF)
-    found = int((res != none).sum())
-    even_found = bool((res[0::2] != none).all())
-    log(f"answers equal the plain search: {exact}; found {found}/{Q}; every sampled k-mer found: {even_found}")
-    ix.counters(reset=True)
-    ix.search_kmers(kmers, out=out, count=True)
-    torch.cuda.synchronize()
-    lines = int(ix.counters(reset=True)["lines"])
-    kern_s = results["plain"]["search_kernel_ms"] / 1e3
-    achieved = lines * 128 / kern_s / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None, "kernel": "k_search_kmers",
-                "kernel_avg_ms": results["plain"]["search_kernel_ms"], "lines_per_launch": lines,
-                "bytes_per_kmer": round(lines * 128 / Q, 2), "index_bytes": info["device_bytes"]}
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline_kmers(ix, kmers, res, k, args)
-    checkpoint = image_round_trip(ix, kmers, out, res) if args.checkpoint else None  # closes ix
-    line = {
-        "metric": METRIC, "value": results["plain"]["kmers_per_s"], "unit": "k-mer queries/s", "n_gpus": 1,
-        "steps": nsteps, "warmup": args.warmup, "ms_per_step": results["plain"]["ms_per_step"],
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32 search, bf16/fp32 MLP",
-        "data": "synthetic (seeded, generated on the GPU)",
-        "config": {"workload": f"pangenome index: {args.haplotypes} haplotypes of a {args.base_bp / 1e6:g} Mbp "
-                               f"uniform base genome (0.1% SNP, 0.01% indel), k={k}, presearch {args.presearch}; "
-                               f"{Q} independent 31-mers, 50% present; plain and prefilter side by side",
-                   "bases": G, "index_columns": info["n"], "index_build_s": round(build_s, 1),
-                   "layout": f"L{info['layout']}", "found_fraction": round(found / Q, 4),
-                   "paths": results, "answers_equal_plain": exact, "sampled_kmers_all_found": even_found,
-                   "filtered_vs_plain": round(results["filtered"]["kmers_per_s"] / results["plain"]["kmers_per_s"], 3),
-                   "bloom_only_vs_plain": round(results["bloom_only"]["kmers_per_s"] /
-                                                results["plain"]["kmers_per_s"], 3),
-                   "bloom_only_bytes": bloom_all.bytes, "train": pf.stats, "train_total_s": round(train_total_s, 1)},
-        "roofline": roofline, "cpu_baseline": cpu,
-    }
-    print(json.dumps(line), flush=True)
-    return 0
-
-
 def cpu_baseline_rows(ix, chars, starts_h, res_gpu, k, args):
     """The CPU oracle's read search on the first rows of the batch + parity."""
     from oracle import oracle as O
@@ -1166,9 +1276,11 @@ def measure_traffic(args):
     prof = shutil.which("rocprofv3")
     if not prof:
         return {"error": "rocprofv3 not found"}
-    kern = "k_search_kmers" if args.workload == "large" else (
+    large = args.workload in ("large", "uniform")
+    kern = "k_search_kmers" if large else (
         "k_reads_uniform" if args.mode in ("default", "persistent", "persistent-nocert", "persistent-independent") else "k_search_reads")
-    child = [sys.executable, "-u", os.path.abspath(__file__), "--pmc-child", "--workload", args.workload,
+    child = [sys.executable, "-u", os.path.abspath(__file__), "--pmc-child",
+             "--workload", "uniform" if large else args.workload,
              "--mode", args.mode, "--layout", str(args.layout), "--presearch", str(args.presearch),
              "--seg", str(args.seg), "--genome-bp", str(args.genome_bp), "--reads", str(args.reads),
              "--read-len", str(args.read_len), "--k", str(args.k), "--large-bp", str(args.large_bp),
@@ -1189,35 +1301,42 @@ def measure_traffic(args):
             return {"error": f"{tag} run produced no counters (rc {r.returncode}): {r.stderr[-300:]}"}
         copy, search = {c: [] for c in ctrs}, {c: [] for c in ctrs}
         with open(files[0]) as fh:
-            for row in csv.DictReader(fh):
-                name, c, v = row["Kernel_Name"], row.get("Counter_Name", tag), float(row["Counter_Value"])
-                if c not in search:
-                    continue
-                if "__amd_rocclr_copyBuffer" in name:
-                    copy[c].append(v)
-                elif kern in name and ", true>" not in name.split("(")[0][-8:]:
-                    search[c].append(v)
+            rows_ = list(csv.DictReader(fh))
+        if rows_ and "Dispatch_Id" in rows_[0]:
+            rows_.sort(key=lambda r: int(r["Dispatch_Id"]))
+        for row in rows_:
+            name, c, v = row["Kernel_Name"], row.get("Counter_Name", tag), float(row["Counter_Value"])
+            if c not in search:
+                continue
+            if "__amd_rocclr_copyBuffer" in name:
+                copy[c].append(v)
+            elif kern in name and ", true>" not in name.split("(")[0][-8:]:
+                search[c].append(v)
         shutil.rmtree(d, ignore_errors=True)
         if not all(search.values()):
             return {"error": f"{tag}: search kernel missing from the counters"}
+        # the median dispatch: the build's own check also launches the search
+        # kernel, on a few million sampled k-mers
+        med = {c: float(np.median(v)) for c, v in search.items()}
         if len(ctrs) == 2:  # the L2 hit rate: a ratio, no calibration
-            hit = sum(search[ctrs[0]]) / len(search[ctrs[0]])
-            miss = sum(search[ctrs[1]]) / len(search[ctrs[1]])
+            hit, miss = med[ctrs[0]], med[ctrs[1]]
             out["tcc"] = {"hit_per_dispatch": round(hit), "miss_per_dispatch": round(miss),
                           "hit_rate": round(hit / max(hit + miss, 1.0), 4)}
             continue
         if not copy[tag]:
             return {"error": f"{tag}: copy kernel missing from the counters"}
-        # the three 1 GiB copies are the largest copy dispatches of the child
-        big = sorted(copy[tag])[-3:]
+        # the three 1 GiB copies are the child's first copy dispatches (it
+        # makes them before anything else; later copies, such as tensor
+        # clones, have other sizes)
+        big = copy[tag][:3]
         copy_kib = sum(big) / len(big)
         if copy_kib < 100_000:
             return {"error": f"{tag}: the 1 GiB calibration copies were not found"}
         factor = (1 << 20) / copy_kib
-        kib = sum(search[tag]) / len(search[tag])
+        kib = med[tag]
         out["counters"][tag] = {"search_KiB_per_dispatch": round(kib, 1), "copy_KiB": round(copy_kib, 1),
                                 "scale": round(factor, 4), "bytes": round(kib * 1024 * factor)}
-    out["queries_per_launch"] = min(args.queries, 250_000_000) if args.workload == "large" else None
+    out["queries_per_launch"] = min(args.queries, 250_000_000) if large else None
     return out
 
 

@@ -447,10 +447,13 @@ def format_results(results: np.ndarray, offsets: np.ndarray) -> bytes:
     return buf.raw[: size.value]
 
 
-def _flags(n_as_a=False, independent=False, count=False, persistent=False, certificates=True, lanes=False) -> int:
+def _flags(n_as_a=False, independent=False, count=False, persistent=False, certificates=True, lanes=False,
+           runs=False) -> int:
+    """runs: work items are the rows' N-free runs (plans only; see SBWT_READS_RUN_ITEMS)."""
     return ((_capi.READS_N_AS_A if n_as_a else 0) | (_capi.READS_INDEPENDENT if independent else 0)
             | (_capi.COUNT if count else 0) | (_capi.READS_PERSISTENT if persistent else 0)
-            | (0 if certificates else _capi.READS_NO_CERT) | (_capi.READS_SEGMENT_LANES if lanes else 0))
+            | (0 if certificates else _capi.READS_NO_CERT) | (_capi.READS_SEGMENT_LANES if lanes else 0)
+            | (_capi.READS_RUN_ITEMS if runs else 0))
 
 
 class ReadsPlan:

@@ -2,9 +2,10 @@
 
 Times the read search of a sample batch under every combination of rank
 layout (16/32/64-byte chunks), presearch depth, segment length (k-mers per
-work item) and kernel (one lane per segment, or the persistent uniform-step
-kernel), checks that every combination gives the same answers, and leaves
-the index in the fastest layout and depth. The reference fixes these by
+work item), kernel (one lane per work item, or the persistent uniform-step
+kernel) and work items (fixed segments of rows, or the rows' N-free runs),
+checks that every combination gives the same answers, and leaves the index
+in the fastest layout and depth. The reference fixes these by
 flags (superblock size, presearch depth and threads per block,
 include/parameters.hpp:39-58); here they are measured.
 """
@@ -18,8 +19,19 @@ import torch
 from .api import Index, ReadsPlan
 
 
+KERNELS = ("lane", "persistent", "lane-runs", "persistent-runs")
+
+
+def kernel_flags(kernel: str) -> dict:
+    """Search flags of a kernel name: one lane per work item ("lane") or the
+    persistent uniform-step kernel ("persistent"); "-runs": the work items
+    are the rows' N-free runs instead of fixed segments."""
+    return {"persistent": kernel.startswith("persistent"), "lanes": kernel.startswith("lane"),
+            "runs": kernel.endswith("-runs")}
+
+
 def autotune(index: Index, chars: torch.Tensor, row_starts: torch.Tensor, *, layouts=(16, 32, 64),
-             depths=(10, 11, 12, 13), segs=(64, 256, 1024), kernels=("lane", "persistent"), reps: int = 3,
+             depths=(10, 11, 12, 13), segs=(64, 256, 1024), kernels=KERNELS, reps: int = 3,
              max_kmers: int | None = None, budget_s: float = 60.0, n_as_a: bool = False, log=None) -> dict:
     """chars / row_starts: a sample batch on the device (row_starts int64,
     offsets into chars). Returns {"best": {...}, "trials": [...]} and sets
@@ -43,7 +55,7 @@ def autotune(index: Index, chars: torch.Tensor, row_starts: torch.Tensor, *, lay
             index.presearch(m)
         for seg, kern in itertools.product(segs, kernels):
             plan = ReadsPlan(rows, n_chars, seg)
-            flags = {"persistent": kern == "persistent", "lanes": kern == "lane", "n_as_a": n_as_a}
+            flags = {**kernel_flags(kern), "n_as_a": n_as_a}
             plan.search(index, chars, row_starts, out, **flags)  # warm
             times = []
             for _ in range(reps):

@@ -843,7 +843,11 @@ int build_sbwt(const u8* d_seq, const u64* d_starts, u64 nseq, u32 k, const sbwt
   if (st == SBWT_OK && nsample) {
     DevBuf ans;
     st = dalloc(ans, nsample * 8);
-    if (st == SBWT_OK) st = sbwt_search_kmers(ix, sample.as<u64>(), nsample, ans.as<u64>(), s);
+    // the counted instance, so that the plain kernel's dispatches are the
+    // caller's own; its counts are cleared afterwards
+    if (st == SBWT_OK) st = sbwt_search_kmers_flags(ix, sample.as<u64>(), nsample, ans.as<u64>(), SBWT_COUNT, s);
+    if (st == SBWT_OK && hipMemsetAsync(ix->d_counters, 0, kCounters * sizeof(unsigned long long), s) != hipSuccess)
+      st = fail(SBWT_ERR_HIP, "counter reset failed");
     if (st == SBWT_OK) {
       u64 miss = 0;
       k_count_absent<<<blocks(nsample), 256, 0, s>>>(ans.as<u64>(), nsample, J.acc);

@@ -70,10 +70,32 @@ int verify_groups(const sbwt_index* ix, hipStream_t s);
 int check_groups(const sbwt_index* ix, hipStream_t s);
 // Labels of all nodes as top-aligned k-mers, ~0 / 0 for padded nodes.
 int node_labels(sbwt_index* ix, u64* d_labels, u8* d_full, hipStream_t s);
+// Run items (runs.hip): the N-free runs of k-mers of every row as work
+// items {row, first k-mer, k-mers, 0} of at most seg k-mers, in row order;
+// every window holding a non-ACGT character is answered ~0. Needs
+// k >= kRunItemsMinK, for which run_items_bound items always suffice.
+constexpr u32 kRunItemsMinK = 16;
+u64 run_items_bound(u64 rows, u64 chars, u32 seg);
+// pass 1: items per row into row_items[rows]; pass 2 (after an exclusive
+// scan of them into row_base[rows + 1]): the items, and ~0 for N windows
+int run_item_counts(const u64* nmask, const u64* row_starts, const u64* offsets, u64 rows, u32 k, u32 seg,
+                    u64* row_items, hipStream_t s);
+int run_item_write(const u64* nmask, const u64* row_starts, const u64* offsets, u64 rows, u32 k, u32 seg, u64* out,
+                   uint4* runs, u64 cap, const u64* row_base, hipStream_t s);
+// Work items of a read search over run items (null runs: segments of the
+// work prefix).
+struct RunList {
+  const uint4* runs = nullptr;
+  const unsigned long long* count = nullptr;
+  u64 cap = 0;
+};
 // The persistent read search (stream.hip).
 int search_reads_persistent(const sbwt_index* ix, const u64* packed, const u64* nmask, const u64* rs,
                             const u64* off, const u64* work, const u32* items, u64 rows, u64 bound, u32 seg,
                             const u8* maybe, u64* out, bool nbreak, bool stream, bool cert, bool count,
-                            hipStream_t s);
+                            const RunList& runs, hipStream_t s);
+// The read search with run items, as sbwt_search_reads with flags (search.hip).
+int search_reads_runs(const sbwt_index* ix, const u64* packed, const u64* row_starts, const u64* offsets, u64 rows,
+                      u32 seg, const u8* maybe, u64* out, u32 flags, const RunList& runs, hipStream_t s);
 
 }  // namespace sbwt

@@ -212,23 +212,40 @@ __global__ __launch_bounds__(256) void k_search_reads(DevIndex ix, const u64* __
                                                       const u64* __restrict__ offsets,
                                                       const u64* __restrict__ work, u64 rows, u64 bound, u32 seg,
                                                       const u8* __restrict__ maybe, u64* __restrict__ out,
-                                                      u32 vec_out, unsigned long long* ctr) {
+                                                      u32 vec_out, unsigned long long* ctr,
+                                                      const uint4* __restrict__ runs,
+                                                      const unsigned long long* __restrict__ run_count) {
   __shared__ u64 obuf[kAnswerSlots * kAnswerStride];
   AnswerBuffer ab{obuf + threadIdx.x};
   const u64 gid = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x;
   Tally t;
   u32 done = 0;
-  if (gid < bound && rows != 0 && gid < work[rows]) {
-    // the row: work[r] <= gid < work[r+1]
+  // the item: k-mers [j0, j0 + cnt) of row r, from a run item or from the
+  // work prefix (work[r] <= gid < work[r+1], segments of seg k-mers)
+  u64 r = 0, j0 = 0, cnt = 0;
+  bool have = false;
+  if (runs) {
+    have = gid < bound && gid < *run_count;
+    if (have) {
+      const uint4 it = runs[gid];
+      r = it.x;
+      j0 = it.y;
+      cnt = it.z;
+    }
+  } else if (gid < bound && rows != 0 && gid < work[rows]) {
+    have = true;
     u64 lo = 0, hi = rows;
     while (hi - lo > 1) {
       const u64 mid = (lo + hi) >> 1;
       if (work[mid] <= gid) lo = mid; else hi = mid;
     }
-    const u64 r = lo;
+    r = lo;
+    const u64 rk = offsets[r + 1] - offsets[r];
+    j0 = (gid - work[r]) * seg;
+    cnt = (rk - j0) < seg ? (rk - j0) : seg;
+  }
+  if (have) {
     const u64 rowk = offsets[r + 1] - offsets[r];
-    const u64 j0 = (gid - work[r]) * seg;
-    const u64 cnt = (rowk - j0) < seg ? (rowk - j0) : seg;
     const u64 rs = row_starts[r];
     const u64 obase = offsets[r];
     const u64 nk = maybe ? offsets[rows] : 0;
@@ -354,6 +371,7 @@ struct ReadArgs {
   u32 seg;
   const u8* maybe;
   u64* out;
+  RunList runs;
 };
 
 template <int CB, bool STREAM, bool NBREAK, bool CERT, bool COUNT>
@@ -361,7 +379,7 @@ static int launch_reads5(const sbwt_index* ix, const ReadArgs& a, hipStream_t s)
   const u32 bt = kAnswerStride;  // the answer buffer's LDS layout assumes this block size
   k_search_reads<CB, STREAM, NBREAK, CERT, COUNT><<<(a.bound + bt - 1) / bt, bt, 0, s>>>(
       dev_view(ix), a.packed, a.nmask, a.rs, a.off, a.work, a.rows, a.bound, a.seg, a.maybe, a.out,
-      (reinterpret_cast<uintptr_t>(a.out) & 15) == 0 ? 1u : 0u, ix->d_counters);
+      (reinterpret_cast<uintptr_t>(a.out) & 15) == 0 ? 1u : 0u, ix->d_counters, a.runs.runs, a.runs.count);
   SBWT_HIP(hipGetLastError());
   return SBWT_OK;
 }
@@ -385,6 +403,27 @@ static int launch_reads(const sbwt_index* ix, const ReadArgs& a, bool stream, bo
   if (stream) return launch_reads3<CB, true, false>(ix, a, cert, count, s);
   if (nbreak) return launch_reads3<CB, false, true>(ix, a, cert, count, s);
   return launch_reads3<CB, false, false>(ix, a, cert, count, s);
+}
+
+int search_reads_runs(const sbwt_index* ix, const u64* packed, const u64* row_starts, const u64* offsets, u64 rows,
+                      u32 seg, const u8* maybe, u64* out, u32 flags, const RunList& runs, hipStream_t s) {
+  if (rows == 0 || runs.cap == 0) return SBWT_OK;
+  // run items hold no non-ACGT window: nothing to break on
+  const bool independent = (flags & SBWT_READS_INDEPENDENT) != 0;
+  const bool stream_mode = !independent && ix->has_groups;
+  const bool cert = !independent && !(flags & SBWT_READS_NO_CERT);
+  const bool cnt = (flags & SBWT_COUNT) != 0;
+  const bool uniform = (flags & SBWT_READS_PERSISTENT) ||
+                       (!(flags & SBWT_READS_SEGMENT_LANES) && stream_mode && cert);
+  if (uniform)
+    return search_reads_persistent(ix, packed, nullptr, row_starts, offsets, nullptr, nullptr, rows, runs.cap, seg,
+                                   maybe, out, false, stream_mode, cert, cnt, runs, s);
+  const ReadArgs a{packed, nullptr, row_starts, offsets, nullptr, rows, runs.cap, seg, maybe, out, runs};
+  switch (ix->layout) {
+    case 16: return launch_reads<16>(ix, a, stream_mode, false, cert, cnt, s);
+    case 32: return launch_reads<32>(ix, a, stream_mode, false, cert, cnt, s);
+    default: return launch_reads<64>(ix, a, stream_mode, false, cert, cnt, s);
+  }
 }
 
 }  // namespace sbwt
@@ -474,9 +513,10 @@ static int search_reads_impl(const sbwt_index* ix, const uint64_t* d_packed, con
   }
   if (uniform)
     return search_reads_persistent(ix, d_packed, d_nmask, d_row_starts, d_offsets, d_work, d_items, rows,
-                                   work_bound, seg, d_maybe, d_out, nbreak, stream_mode, cert, cnt,
+                                   work_bound, seg, d_maybe, d_out, nbreak, stream_mode, cert, cnt, RunList{},
                                    as_stream(stream));
-  const ReadArgs a{d_packed, d_nmask, d_row_starts, d_offsets, d_work, rows, work_bound, seg, d_maybe, d_out};
+  const ReadArgs a{d_packed, d_nmask, d_row_starts, d_offsets, d_work, rows, work_bound, seg, d_maybe, d_out,
+                   RunList{}};
   hipStream_t s = as_stream(stream);
   switch (ix->layout) {
     case 16: return launch_reads<16>(ix, a, stream_mode, nbreak, cert, cnt, s);

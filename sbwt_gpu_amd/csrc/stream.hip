@@ -62,6 +62,9 @@ struct ReadsDev {
   u32 seg;
   u32 nbreak, stream, cert;
   u32 vec_out;  // out is 16-byte aligned (full answer groups go out as two 16-byte stores)
+  const uint4* runs;  // run items {row, first k-mer, k-mers}, or null: segments of the work prefix
+  const unsigned long long* run_count;
+  u64 run_cap;  // items the run buffer holds
 };
 
 __device__ __forceinline__ u64 window2(const u64* __restrict__ w, u64 pos) {
@@ -138,7 +141,11 @@ struct Walker {
 
   __device__ __forceinline__ void setup(const ReadsDev& rd, u64 it) {
     u64 row = 0;
-    if (rd.items) {
+    uint4 run = make_uint4(0, 0, 0, 0);
+    if (rd.runs) {
+      run = rd.runs[it];
+      row = run.x;
+    } else if (rd.items) {
       row = rd.items[it];
     } else {
       u64 hi = rd.rows;
@@ -149,8 +156,13 @@ struct Walker {
     }
     const u64 o0 = rd.offsets[row];
     rowk = static_cast<u32>(rd.offsets[row + 1] - o0);
-    j = static_cast<u32>(it - rd.work[row]) * rd.seg;
-    jend = rowk - j < rd.seg ? rowk : j + rd.seg;
+    if (rd.runs) {
+      j = run.y;
+      jend = run.y + run.z;
+    } else {
+      j = static_cast<u32>(it - rd.work[row]) * rd.seg;
+      jend = rowk - j < rd.seg ? rowk : j + rd.seg;
+    }
     cbase = rd.row_starts[row];
     obase = o0;
     prev = false;
@@ -542,7 +554,7 @@ __global__ __launch_bounds__(256) void k_reads_uniform(DevIndex ix, ReadsDev rd,
   __shared__ u64 obuf[answer_slots<CB>() * kAnswerStride];
   typename W::Ans ab{obuf + threadIdx.x};
   const u32 lane = threadIdx.x & 63;
-  const u64 total_items = rd.rows ? rd.work[rd.rows] : 0;
+  const u64 total_items = rd.runs ? min(static_cast<u64>(*rd.run_count), rd.run_cap) : (rd.rows ? rd.work[rd.rows] : 0);
   const u64 nk = rd.maybe && rd.rows ? rd.offsets[rd.rows] : 0;
   W w[P];
   UTally t;
@@ -613,10 +625,10 @@ static int launch_uniform(const sbwt_index* ix, const ReadsDev& rd, u64 bound, u
 int search_reads_persistent(const sbwt_index* ix, const u64* packed, const u64* nmask, const u64* rs,
                             const u64* off, const u64* work, const u32* items, u64 rows, u64 bound, u32 seg,
                             const u8* maybe, u64* out, bool nbreak, bool stream, bool cert, bool count,
-                            hipStream_t s) {
+                            const RunList& runs, hipStream_t s) {
   const ReadsDev rd{packed, nmask, rs, off, work, items, maybe, rows, seg,
                     nbreak ? 1u : 0u, stream ? 1u : 0u, cert ? 1u : 0u,
-                    (reinterpret_cast<uintptr_t>(out) & 15) == 0 ? 1u : 0u};
+                    (reinterpret_cast<uintptr_t>(out) & 15) == 0 ? 1u : 0u, runs.runs, runs.count, runs.cap};
   switch (ix->layout) {
     case 16: return count ? launch_uniform<16, true>(ix, rd, bound, out, s) : launch_uniform<16, false>(ix, rd, bound, out, s);
     case 32: return count ? launch_uniform<32, true>(ix, rd, bound, out, s) : launch_uniform<32, false>(ix, rd, bound, out, s);

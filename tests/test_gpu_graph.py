@@ -89,9 +89,129 @@ def test_autotune_keeps_answers_and_applies_best():
     rng, s, ix, oix, chars, starts, set_starts = _setup(seed=3)
     res = autotune.autotune(ix, chars, starts, layouts=(16, 32), depths=(8, 10), segs=(64, 256), reps=1)
     best = res["best"]
-    assert len(res["trials"]) == 2 * 2 * 2 * 2
+    assert len(res["trials"]) == 2 * 2 * 2 * len(autotune.KERNELS)
     assert ix.info["layout"] == best["layout"] and ix.info["presearch_depth"] == best["presearch_depth"]
-    batch = sg.ReadBatch(chars, starts, 31, seg=best["segment_kmers"])
     want = O.search_reads(oix, chars.cpu().numpy(), starts.cpu().numpy().astype(np.uint64), 31)
-    np.testing.assert_array_equal(sg.to_u64(ix.search_reads(batch, persistent=best["kernel"] == "persistent",
-                                                            lanes=best["kernel"] == "lane")), want)
+    plan = sg.ReadsPlan(starts.numel() - 1, chars.numel(), seg=best["segment_kmers"])
+    out = torch.empty(want.size, dtype=torch.int64, device="cuda")
+    plan.search(ix, chars, starts, out, **autotune.kernel_flags(best["kernel"]))
+    np.testing.assert_array_equal(sg.to_u64(out), want)
+
+
+# ---------------------------------------------------------------------------
+# run items (SBWT_READS_RUN_ITEMS): work items are the rows' N-free runs
+
+
+def _ragged_rows(rng, g, k):
+    """Rows that reach every case of the run pass: shorter than k, exactly k,
+    all N, no N at all (longer than 64 windows and than a segment), N at
+    window-chunk boundaries (window 63, 64, 127), N in the first and last
+    character, dense N, and lowercase / other non-ACGT bytes."""
+    def piece(n):
+        p = int(rng.integers(0, len(g) - n))
+        return g[p:p + n]
+    rows = [piece(k - 1), piece(k), "N" * 80, piece(2000), piece(700), "N" + piece(300), piece(300) + "N",
+            piece(200).lower(), piece(90) + "RYKM" + piece(90), ""]
+    for pos in (63, 64, 127, 128, 63 + k - 1, 64 + k - 1):
+        r = list(piece(400))
+        r[pos] = "N"
+        rows.append("".join(r))
+    for rate in (0.05, 0.2, 0.5):
+        r = np.array(list(piece(1500)))
+        r[rng.random(r.size) < rate] = "N"
+        rows.append("".join(r))
+    r = list(piece(500))  # a sequencing error inside a run, so certificates fire
+    r[250] = "ACGT"[("ACGT".index(r[250]) + 1) % 4]
+    rows.append("".join(r))
+    return rows
+
+
+@pytest.mark.parametrize("seg", [17, 64, 256])
+@pytest.mark.parametrize("kernel", ["persistent", "lane"])
+def test_run_items_match_oracle(seg, kernel):
+    rng, s, ix, oix, chars, starts, set_starts = _setup(seed=5)
+    k = 31
+    gtxt = synth.to_ascii(synth.genome_codes(150_000, 5, gc=0.45)).tobytes().decode()
+    rows = _ragged_rows(rng, gtxt, k)
+    ch, rs = O.parse_lines("\n".join(rows).encode())
+    # parse_lines skips empty lines; run the batch as the parsed rows
+    want = O.search_reads(oix, ch, rs, k)
+    dch = torch.from_numpy(np.frombuffer(bytes(ch), np.uint8).copy()).cuda()
+    drs = torch.from_numpy(rs.astype(np.int64)).cuda()
+    plan = sg.ReadsPlan(drs.numel() - 1, dch.numel(), seg=seg)
+    out = torch.full((max(want.size, 1),), 7, dtype=torch.int64, device="cuda")
+    flags = {"persistent": kernel == "persistent", "lanes": kernel == "lane"}
+    plan.search(ix, dch, drs, out, runs=True, **flags)
+    np.testing.assert_array_equal(sg.to_u64(out)[: want.size], want)
+    # the N-heavy read sets of the batched workload, batch by batch
+    sh = starts.cpu().numpy()
+    kc = np.maximum(np.diff(sh) - k + 1, 0)
+    kcum = np.concatenate([[0], np.cumsum(kc)])
+    want2 = O.search_reads(oix, chars.cpu().numpy(), sh.astype(np.uint64), k)
+    batches = _batches(chars, starts, set_starts)
+    plan2 = sg.ReadsPlan(max(b[1].numel() - 1 for b in batches), max(b[0].numel() for b in batches), seg=seg)
+    out2 = torch.full((int(kcum[-1]),), 7, dtype=torch.int64, device="cuda")
+    for bc, brel, b0, b1 in batches:
+        plan2.search(ix, bc, brel, out2[int(kcum[b0]):int(kcum[b1])], runs=True, **flags)
+    np.testing.assert_array_equal(sg.to_u64(out2), want2)
+    # counted: the same answers
+    ix.counters(reset=True)
+    plan.search(ix, dch, drs, out, runs=True, count=True, **flags)
+    assert ix.counters(reset=True)["kmers"] <= want.size
+    np.testing.assert_array_equal(sg.to_u64(out)[: want.size], want)
+
+
+def test_run_items_graph_replays():
+    rng, s, ix, oix, chars, starts, set_starts = _setup(seed=6)
+    k = 31
+    sh = starts.cpu().numpy()
+    kc = np.maximum(np.diff(sh) - k + 1, 0)
+    kcum = np.concatenate([[0], np.cumsum(kc)])
+    want = O.search_reads(oix, chars.cpu().numpy(), sh.astype(np.uint64), k)
+    batches = _batches(chars, starts, set_starts)
+    plan = sg.ReadsPlan(max(b[1].numel() - 1 for b in batches), max(b[0].numel() for b in batches), seg=64)
+    out = torch.empty(int(kcum[-1]), dtype=torch.int64, device="cuda")
+
+    def run_all(stream=None):
+        for bc, brel, b0, b1 in batches:
+            plan.search(ix, bc, brel, out[int(kcum[b0]):int(kcum[b1])], stream=stream, runs=True)
+
+    run_all()
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(sg.to_u64(out), want)
+    eager = out.clone()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    gr = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(gr, stream=side):
+        run_all(stream=side)
+    torch.cuda.synchronize()
+    for rep in range(4):
+        out.fill_(-7)
+        torch.cuda.synchronize()
+        _ = (torch.arange(1000, device="cuda") * rep).sum().item()
+        gr.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, eager), f"replay {rep}"
+
+
+def test_run_items_ignored_where_unsupported():
+    """k < 16 or N coded as A: the plan searches segments as before."""
+    rng = np.random.default_rng(7)
+    g = "".join(ACGT[rng.integers(0, 4, 20_000)])
+    s = O.SBWT([g], 11)
+    ix = sg.Index.from_rows(s.rows, s.n, 11, group=s.group, config=sg.Config(presearch_depth=6))
+    rows = [g[p:p + 120] if i % 2 else g[p:p + 60] + "N" + g[p + 61:p + 120]
+            for i, p in enumerate(rng.integers(0, 19_000, 50))]
+    ch, rs = O.parse_lines("\n".join(rows).encode())
+    oix = O.Index(s.rows, s.n)
+    oix.presearch(6)
+    want = O.search_reads(oix, ch, rs, 11)
+    dch = torch.from_numpy(np.frombuffer(bytes(ch), np.uint8).copy()).cuda()
+    drs = torch.from_numpy(rs.astype(np.int64)).cuda()
+    plan = sg.ReadsPlan(drs.numel() - 1, dch.numel(), seg=32)
+    out = torch.empty(want.size, dtype=torch.int64, device="cuda")
+    plan.search(ix, dch, drs, out, runs=True)
+    np.testing.assert_array_equal(sg.to_u64(out), want)
+    plan.search(ix, dch, drs, out, runs=True, n_as_a=True)
+    np.testing.assert_array_equal(sg.to_u64(out), O.search_reads(oix, ch, rs, 11, n_breaks=False))
