@@ -100,6 +100,8 @@ def main() -> int:
     ap.add_argument("--sets", type=int, default=10_000)
     ap.add_argument("--batch-chars", type=int, default=256 << 20)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--run-items", action="store_true",
+                    help="batched with --no-autotune: work items are the rows' N-free runs (else segments)")
     ap.add_argument("--train-positives", type=int, default=50_000_000)
     ap.add_argument("--train-negatives", type=int, default=50_000_000)
     ap.add_argument("--train-batch", type=int, default=65536)
@@ -742,7 +744,8 @@ def run_batched(args) -> int:
 
     tune = None
     seg = args.seg
-    kflags = autotune.kernel_flags("persistent" if args.mode.startswith("persistent") else "lane")
+    kflags = autotune.kernel_flags(("persistent" if args.mode.startswith("persistent") else "lane") +
+                                   ("-runs" if args.run_items else ""))
     if not args.no_autotune:
         sample_chars, sample_rel, _, _ = batches[0]
         t = time.time()

@@ -63,12 +63,13 @@ extern "C" {
 #define SBWT_READS_RUN_ITEMS 64u  /* sbwt_reads_plan_search only: work items are
                                      the rows' runs of k-mers free of non-ACGT
                                      characters (pieces of at most seg k-mers),
-                                     found by a wave-cooperative pass that also
-                                     answers every window holding such a
-                                     character; the search kernels then skip
-                                     all non-ACGT checks. Ignored for k < 16,
-                                     with SBWT_READS_N_AS_A, and for batches of
-                                     2^32 - 64 characters or more */
+                                     each led by the gap of windows holding
+                                     such a character before it, found by a
+                                     pass over the rows' non-ACGT mask; the
+                                     search kernels answer a gap in one step
+                                     and never test a character. Ignored for
+                                     k < 16, with SBWT_READS_N_AS_A, and for
+                                     batches of 2^32 - 64 characters or more */
 #define SBWT_COUNT 4u             /* counted variant: add the 128-byte lines the
                                      rank reads touch to the index's counters
                                      (sbwt_counters); also accepted by

@@ -70,18 +70,17 @@ int verify_groups(const sbwt_index* ix, hipStream_t s);
 int check_groups(const sbwt_index* ix, hipStream_t s);
 // Labels of all nodes as top-aligned k-mers, ~0 / 0 for padded nodes.
 int node_labels(sbwt_index* ix, u64* d_labels, u8* d_full, hipStream_t s);
-// Run items (runs.hip): the N-free runs of k-mers of every row as work
-// items {row, first k-mer, k-mers, 0} of at most seg k-mers, in row order;
-// every window holding a non-ACGT character is answered ~0. Needs
+// Run items (runs.hip): work items {row, a, b, c} that tile every row:
+// windows [a, b) hold a non-ACGT character (answered ~0 by
+// run_items before the search, which skips them), [b, c) are an N-free run
+// of at most seg k-mers. Needs
 // k >= kRunItemsMinK, for which run_items_bound items always suffice.
 constexpr u32 kRunItemsMinK = 16;
 u64 run_items_bound(u64 rows, u64 chars, u32 seg);
-// pass 1: items per row into row_items[rows]; pass 2 (after an exclusive
-// scan of them into row_base[rows + 1]): the items, and ~0 for N windows
-int run_item_counts(const u64* nmask, const u64* row_starts, const u64* offsets, u64 rows, u32 k, u32 seg,
-                    u64* row_items, hipStream_t s);
-int run_item_write(const u64* nmask, const u64* row_starts, const u64* offsets, u64 rows, u32 k, u32 seg, u64* out,
-                   uint4* runs, u64 cap, const u64* row_base, hipStream_t s);
+// the items, their number in *count (device), and out[0 .. offsets[rows]) = ~0
+// (the answers of the gaps; the search writes the runs' answers over it)
+int run_items(const u64* nmask, const u64* row_starts, const u64* offsets, u64 rows, u32 k, u32 seg, uint4* runs,
+              u64 cap, unsigned long long* count, u64* out, hipStream_t s);
 // Work items of a read search over run items (null runs: segments of the
 // work prefix).
 struct RunList {

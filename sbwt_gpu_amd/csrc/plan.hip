@@ -23,8 +23,7 @@ struct sbwt_reads_plan {
   size_t scan_bytes = 0;
   // run items (SBWT_READS_RUN_ITEMS): capacity for any batch of the plan at k >= 16
   uint4* d_runs = nullptr;
-  sbwt::u64* d_run_items = nullptr;  // items per row, then (scanned) the rows' first items
-  sbwt::u64* d_run_base = nullptr;
+  unsigned long long* d_run_count = nullptr;
   sbwt::u64 run_cap = 0;
 };
 
@@ -41,13 +40,6 @@ struct PlanKmerCount {
     const u64 len = rs[r + 1] - rs[r];
     return len >= k ? len - k + 1 : 0;
   }
-};
-
-struct PlanRunCount {
-  const u64* items;
-  u64 rows;
-  u32 unused;
-  __host__ __device__ u64 operator()(u64 r) const { return r < rows ? items[r] : 0; }
 };
 
 struct PlanWorkCount {
@@ -104,8 +96,6 @@ int sbwt_reads_plan_create(uint64_t max_rows, uint64_t max_chars, uint32_t seg, 
   p->seg = seg;
   const u64 bound = sbwt_reads_work_bound(max_rows, max_chars, seg);
   size_t b1 = scan_bytes_for<PlanKmerCount>(max_rows + 1), b2 = scan_bytes_for<PlanWorkCount>(max_rows + 1);
-  const size_t b3 = scan_bytes_for<PlanRunCount>(max_rows + 1);
-  if (b3 > b2) b2 = b3;
   p->scan_bytes = (b1 > b2 ? b1 : b2) + 256;
   if (hipMalloc(&p->d_packed, sbwt_packed_words(max_chars) * 8) != hipSuccess ||
       hipMalloc(&p->d_nmask, sbwt_nmask_words(max_chars) * 8) != hipSuccess ||
@@ -113,8 +103,7 @@ int sbwt_reads_plan_create(uint64_t max_rows, uint64_t max_chars, uint32_t seg, 
       hipMalloc(&p->d_work, (max_rows + 1) * 8) != hipSuccess ||
       hipMalloc(&p->d_items, bound * 4) != hipSuccess || hipMalloc(&p->d_scan_tmp, p->scan_bytes) != hipSuccess ||
       hipMalloc(&p->d_runs, (p->run_cap = run_items_bound(max_rows, max_chars, seg)) * sizeof(uint4)) != hipSuccess ||
-      hipMalloc(&p->d_run_items, (max_rows + 1) * 8) != hipSuccess ||
-      hipMalloc(&p->d_run_base, (max_rows + 1) * 8) != hipSuccess) {
+      hipMalloc(&p->d_run_count, sizeof(unsigned long long)) != hipSuccess) {
     sbwt_reads_plan_free(p);
     return fail(SBWT_ERR_HIP, "out of device memory for a plan of %llu rows, %llu characters",
                 (unsigned long long)max_rows, (unsigned long long)max_chars);
@@ -127,8 +116,7 @@ int sbwt_reads_plan_free(sbwt_reads_plan* p) {
   if (!p) return SBWT_OK;
   for (void* q : {static_cast<void*>(p->d_packed), static_cast<void*>(p->d_nmask), static_cast<void*>(p->d_offsets),
                   static_cast<void*>(p->d_work), static_cast<void*>(p->d_items), p->d_scan_tmp,
-                  static_cast<void*>(p->d_runs), static_cast<void*>(p->d_run_items),
-                  static_cast<void*>(p->d_run_base)})
+                  static_cast<void*>(p->d_runs), static_cast<void*>(p->d_run_count)})
     if (q) (void)hipFree(q);
   delete p;
   return SBWT_OK;
@@ -150,16 +138,12 @@ int sbwt_reads_plan_search(sbwt_reads_plan* p, const sbwt_index* ix, const uint8
   SBWT_TRY(plan_scan(p, PlanKmerCount{d_row_starts, rows, ix->k}, rows, off, s));
   if ((flags & SBWT_READS_RUN_ITEMS) && !(flags & SBWT_READS_N_AS_A) && ix->k >= kRunItemsMinK &&
       n_chars < 0xFFFFFFC0ull) {
-    // the rows' N-free runs as work items (the second pass answers every N
-    // window); the item count is the scan's last entry
-    SBWT_TRY(run_item_counts(p->d_nmask, d_row_starts, off, rows, ix->k, p->seg, p->d_run_items, s));
-    SBWT_TRY(plan_scan(p, PlanRunCount{p->d_run_items, rows, 0}, rows, p->d_run_base, s));
-    SBWT_TRY(run_item_write(p->d_nmask, d_row_starts, off, rows, ix->k, p->seg, d_out, p->d_runs, p->run_cap,
-                            p->d_run_base, s));
+    // the rows' N-free runs, each with the gap of N windows before it, as
+    // work items; the same pass fills every answer with ~0 first
+    SBWT_TRY(run_items(p->d_nmask, d_row_starts, off, rows, ix->k, p->seg, p->d_runs, p->run_cap, p->d_run_count,
+                       d_out, s));
     return search_reads_runs(ix, p->d_packed, d_row_starts, off, rows, p->seg, nullptr, d_out, flags,
-                             RunList{p->d_runs, reinterpret_cast<const unsigned long long*>(p->d_run_base + rows),
-                                     p->run_cap},
-                             s);
+                             RunList{p->d_runs, p->d_run_count, p->run_cap}, s);
   }
   SBWT_TRY(plan_scan(p, PlanWorkCount{off, rows, p->seg}, rows, p->d_work, s));
   k_plan_items<<<(rows + 255) / 256, 256, 0, s>>>(p->d_work, rows, p->d_items);

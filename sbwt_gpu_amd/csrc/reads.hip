@@ -47,11 +47,28 @@ __global__ __launch_bounds__(256) void k_pack(const u8* __restrict__ chars, u64 
   if (q >= threads) return;
   const u64 first = q * 32;
   u32 w[8];
-  if (first + 32 <= n && (reinterpret_cast<uintptr_t>(chars + first) & 15) == 0) {
+  // the same misalignment for every thread: a batch cut out of a larger
+  // buffer starts anywhere
+  const u32 off = static_cast<u32>(reinterpret_cast<uintptr_t>(chars) & 15);
+  const uintptr_t end16 = (reinterpret_cast<uintptr_t>(chars + n) + 15) & ~static_cast<uintptr_t>(15);
+  if (first + 32 <= n && off == 0) {
     const uint4* p = reinterpret_cast<const uint4*>(chars + first);
     const uint4 v0 = p[0], v1 = p[1];
     w[0] = v0.x; w[1] = v0.y; w[2] = v0.z; w[3] = v0.w;
     w[4] = v1.x; w[5] = v1.y; w[6] = v1.z; w[7] = v1.w;
+  } else if (first + 32 <= n && reinterpret_cast<uintptr_t>(chars + first - off) + 48 <= end16) {
+    // three aligned 16-byte loads (the last one ends in the 16-byte block
+    // that holds the batch's last byte) and byte funnel shifts
+    const uint4* p = reinterpret_cast<const uint4*>(chars + first - off);
+    const uint4 v0 = p[0], v1 = p[1], v2 = p[2];
+    const u32 W[12] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, v2.z, v2.w};
+    const u32 wo = off >> 2, bo = off & 3;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const u32 lo = wo == 0 ? W[i] : (wo == 1 ? W[i + 1] : (wo == 2 ? W[i + 2] : W[i + 3]));
+      const u32 hi = wo == 0 ? W[i + 1] : (wo == 1 ? W[i + 2] : (wo == 2 ? W[i + 3] : W[i + 4]));
+      w[i] = bo ? __builtin_amdgcn_alignbyte(hi, lo, bo) : lo;
+    }
   } else {
 #pragma unroll
     for (int i = 0; i < 8; i++) {

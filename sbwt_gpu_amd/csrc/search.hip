@@ -222,7 +222,7 @@ __global__ __launch_bounds__(256) void k_search_reads(DevIndex ix, const u64* __
   u32 done = 0;
   // the item: k-mers [j0, j0 + cnt) of row r, from a run item or from the
   // work prefix (work[r] <= gid < work[r+1], segments of seg k-mers)
-  u64 r = 0, j0 = 0, cnt = 0;
+  u64 r = 0, j0 = 0, cnt = 0, jv = 0;  // jv: first k-mer after the run item's leading gap
   bool have = false;
   if (runs) {
     have = gid < bound && gid < *run_count;
@@ -230,7 +230,8 @@ __global__ __launch_bounds__(256) void k_search_reads(DevIndex ix, const u64* __
       const uint4 it = runs[gid];
       r = it.x;
       j0 = it.y;
-      cnt = it.z;
+      jv = it.z;
+      cnt = it.w - it.y;
     }
   } else if (gid < bound && rows != 0 && gid < work[rows]) {
     have = true;
@@ -260,7 +261,9 @@ __global__ __launch_bounds__(256) void k_search_reads(DevIndex ix, const u64* __
     Rolling<2> code{packed};
     Rolling<1> bad{nmask};
     const u64 jend = j0 + cnt;
-    for (u64 j = j0; j < jend; j++) {
+    // a run item's gap [j0, jv): answered ~0 before the search
+    const u64 jfirst = jv > j0 ? (jv < jend ? jv : jend) : j0;
+    for (u64 j = jfirst; j < jend; j++) {
       // a run of k-mers answered ~0 without a search: certified absent,
       // holding a non-ACGT character (every k-mer from j up to the first one
       // holds it), or filtered out

@@ -102,6 +102,7 @@ struct Walker {
   // the item: k-mers [j, jend) of a row of rowk k-mers starting at character cbase
   u64 cbase = 0, obase = 0;
   u32 j = 0, jend = 0, rowk = 0;
+  u32 jv = 0;  // a run item's first k-mer after its leading gap of N windows (0: none)
   bool live = true;
   // the current step
   u32 mode = M_IDLE, purpose = P_KMER;
@@ -158,9 +159,11 @@ struct Walker {
     rowk = static_cast<u32>(rd.offsets[row + 1] - o0);
     if (rd.runs) {
       j = run.y;
-      jend = run.y + run.z;
+      jv = run.z;
+      jend = run.w;
     } else {
       j = static_cast<u32>(it - rd.work[row]) * rd.seg;
+      jv = 0;
       jend = rowk - j < rd.seg ? rowk : j + rd.seg;
     }
     cbase = rd.row_starts[row];
@@ -175,11 +178,14 @@ struct Walker {
 
   // k-mers [from, to) of the row of rk k-mers at characters cb, answers at
   // ob: the second half of another lane's item
-  __device__ __forceinline__ void take(const ReadsDev& rd, u64 cb, u64 ob, u32 rk, u32 from, u32 to) {
+  // (gap_end: the donor's jv, so a half that starts inside a run item's gap
+  // answers the rest of the gap ~0)
+  __device__ __forceinline__ void take(const ReadsDev& rd, u64 cb, u64 ob, u32 rk, u32 from, u32 to, u32 gap_end) {
     cbase = cb;
     obase = ob;
     rowk = rk;
     j = from;
+    jv = gap_end;
     jend = to;
     live = true;
     prev = false;
@@ -289,6 +295,10 @@ struct Walker {
   __device__ __forceinline__ void advance(const DevIndex& ix, const ReadsDev& rd, u64 nk, u64* __restrict__ out,
                                           Ans& ab, UTally& t) {
     const u32 k = ix.k;
+    if (j < jv) {  // a run item's gap: answered ~0 before the search
+      if (COUNT) t.kmers += (jend < jv ? jend : jv) - j;
+      j = jend < jv ? jend : jv;
+    }
     while (mode == M_IDLE && j < jend) {
       // a run of k-mers answered ~0 without a search: certified absent,
       // holding a non-ACGT character (every k-mer from j up to that
@@ -539,8 +549,9 @@ __device__ __forceinline__ void donate(W& w, const ReadsDev& rd, u32 lane) {
   const u32 rk = __shfl(w.rowk, src);
   const u32 md = __shfl(mid, src);
   const u32 je = __shfl(w.jend, src);
+  const u32 jv = __shfl(w.jv, src);
   if (give && grank < np) w.jend = mid;
-  if (recv) w.take(rd, cb, ob, rk, md, je);
+  if (recv) w.take(rd, cb, ob, rk, md, je, jv);
 }
 
 // Each lane walks P work items at once; every iteration issues all P steps'
