@@ -444,6 +444,23 @@ int sbwt_reads_plan_search(sbwt_reads_plan *plan, const sbwt_index *ix,
                            uint64_t *d_out, uint64_t *d_offsets_out,
                            uint32_t flags, void *stream);
 
+/* sbwt_reads_plan_search without a plan: the same query path (flags as
+ * there, seg k-mers per work item, 32..4096, 0 = 256) on a scratch array
+ * the library holds in each GPU's memory for the rest of the process
+ * (about 45 MiB), so that the call allocates nothing. No counterpart in
+ * the reference; it is for callers that cannot allocate device memory:
+ * code under HIP graph capture, tight serving loops. At most
+ * SBWT_STATIC_MAX_ROWS rows and SBWT_STATIC_MAX_CHARS characters per call
+ * (SBWT_ERR_ARG beyond). Every call on one GPU shares that GPU's scratch:
+ * calls that overlap on one GPU race, so order them on one stream. */
+#define SBWT_STATIC_MAX_ROWS 262144ull
+#define SBWT_STATIC_MAX_CHARS 16777216ull
+int sbwt_reads_search_static(const sbwt_index *ix, const uint8_t *d_chars,
+                             uint64_t n_chars, const uint64_t *d_row_starts,
+                             uint64_t rows, uint32_t seg, uint64_t *d_out,
+                             uint64_t *d_offsets_out, uint32_t flags,
+                             void *stream);
+
 /* The found/absent bitmap of count answers: bit (bit_offset + q) of
  * d_bitmap is 1 iff d_results[q] != ~0 (words LSB-first; bits below
  * bit_offset and past the last answer are 0). Writes

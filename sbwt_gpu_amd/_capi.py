@@ -103,6 +103,7 @@ SIGNATURES = {
     "sbwt_reads_plan_create": (i32, [u64, u64, u32, C.POINTER(vp)]),
     "sbwt_reads_plan_free": (i32, [vp]),
     "sbwt_reads_plan_search": (i32, [vp, vp, vp, u64, vp, u64, vp, vp, u32, vp]),
+    "sbwt_reads_search_static": (i32, [vp, vp, u64, vp, u64, u32, vp, vp, u32, vp]),
     "sbwt_last_error": (cp, []),
 }
 

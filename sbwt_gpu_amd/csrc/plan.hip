@@ -5,6 +5,8 @@
 // reference's query pipeline (KmerData::allocate_memory,
 // include/epicseq/kmer_data.hpp:56-69, and DeviceStartPositions,
 // include/epicseq/start_positions.hpp:30-73).
+#include <mutex>
+
 #include <rocprim/device/device_scan.hpp>
 #include <rocprim/iterator/counting_iterator.hpp>
 #include <rocprim/iterator/transform_iterator.hpp>
@@ -122,11 +124,15 @@ int sbwt_reads_plan_free(sbwt_reads_plan* p) {
   return SBWT_OK;
 }
 
-int sbwt_reads_plan_search(sbwt_reads_plan* p, const sbwt_index* ix, const uint8_t* d_chars, uint64_t n_chars,
-                           const uint64_t* d_row_starts, uint64_t rows, uint64_t* d_out, uint64_t* d_offsets_out,
-                           uint32_t flags, void* stream) {
-  clear_error();
-  SBWT_REQUIRE(p && ix, "plan and index must not be NULL");
+}  // extern "C"
+
+namespace sbwt {
+
+// The whole query path of one batch on the scratch of p (a plan, or a view of
+// the static scratch): allocates nothing.
+static int plan_search_impl(sbwt_reads_plan* p, const sbwt_index* ix, const uint8_t* d_chars, uint64_t n_chars,
+                            const uint64_t* d_row_starts, uint64_t rows, uint64_t* d_out, uint64_t* d_offsets_out,
+                            uint32_t flags, void* stream) {
   SBWT_REQUIRE(rows <= p->max_rows && n_chars <= p->max_chars,
                "batch of %llu rows / %llu characters exceeds the plan (%llu / %llu)", (unsigned long long)rows,
                (unsigned long long)n_chars, (unsigned long long)p->max_rows, (unsigned long long)p->max_chars);
@@ -153,4 +159,108 @@ int sbwt_reads_plan_search(sbwt_reads_plan* p, const sbwt_index* ix, const uint8
                            p->seg, d_out, flags, stream);
 }
 
-}  // extern "C"
+}  // namespace sbwt
+
+extern "C" int sbwt_reads_plan_search(sbwt_reads_plan* p, const sbwt_index* ix, const uint8_t* d_chars,
+                                      uint64_t n_chars, const uint64_t* d_row_starts, uint64_t rows, uint64_t* d_out,
+                                      uint64_t* d_offsets_out, uint32_t flags, void* stream) {
+  clear_error();
+  SBWT_REQUIRE(p && ix, "plan and index must not be NULL");
+  return plan_search_impl(p, ix, d_chars, n_chars, d_row_starts, rows, d_out, d_offsets_out, flags, stream);
+}
+
+// ---- the allocation-free variant -----------------------------------------
+//
+// The same query path on a file-scope device array instead of a plan, for
+// callers that cannot allocate (code under HIP graph capture, serving
+// loops). Each GPU holds its own copy of the array until the process ends;
+// in host code its name is not a device address, so its address is taken
+// once per device with hipGetSymbolAddress and kept. Every call on a GPU
+// shares that GPU's array: calls that overlap on one GPU race.
+
+namespace sbwt {
+namespace {
+
+constexpr u64 kStaticRows = SBWT_STATIC_MAX_ROWS;
+constexpr u64 kStaticChars = SBWT_STATIC_MAX_CHARS;
+constexpr u32 kStaticMinSeg = 32;
+constexpr u64 kScanReserve = 1ull << 20;  // rocPRIM's scan temporaries (checked on first use)
+
+constexpr u64 align256(u64 b) { return (b + 255) / 256 * 256; }
+constexpr u64 kPackedBytes = align256((kStaticChars / 32 + 2) * 8);
+constexpr u64 kMaskBytes = align256((kStaticChars / 64 + 2) * 8);
+constexpr u64 kRowBytes = align256((kStaticRows + 1) * 8);
+constexpr u64 kItemBytes = align256((kStaticRows + kStaticChars / kStaticMinSeg + 1) * 4);
+constexpr u64 kRunCap = 2 * kStaticRows + kStaticChars / (kRunItemsMinK + 1) + kStaticChars / kStaticMinSeg + 64;
+constexpr u64 kRunBytes = align256(kRunCap * 16);
+constexpr u64 kScratchBytes =
+    kPackedBytes + kMaskBytes + 2 * kRowBytes + kItemBytes + kScanReserve + kRunBytes + 256;
+
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The calling device's copy of `scratch` (set up on the device's first call).
+int scratch_on_device(void** out) {
+  int dev = 0;
+  SBWT_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(SBWT_ERR_UNSUPPORTED, "device %d: at most %d devices", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    const size_t need = scan_bytes_for<PlanKmerCount>(kStaticRows + 1) > scan_bytes_for<PlanWorkCount>(kStaticRows + 1)
+                            ? scan_bytes_for<PlanKmerCount>(kStaticRows + 1)
+                            : scan_bytes_for<PlanWorkCount>(kStaticRows + 1);
+    if (need > kScanReserve)
+      return fail(SBWT_ERR_UNSUPPORTED, "the scan needs %zu bytes of temporaries, the static scratch holds %llu", need,
+                  (unsigned long long)kScanReserve);
+    void* ptr = nullptr;
+    SBWT_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return SBWT_OK;
+}
+
+}  // namespace
+}  // namespace sbwt
+
+extern "C" int sbwt_reads_search_static(const sbwt_index* ix, const uint8_t* d_chars, uint64_t n_chars,
+                                        const uint64_t* d_row_starts, uint64_t rows, uint32_t seg, uint64_t* d_out,
+                                        uint64_t* d_offsets_out, uint32_t flags, void* stream) {
+  clear_error();
+  SBWT_REQUIRE(ix, "index must not be NULL");
+  if (seg == 0) seg = 256;
+  SBWT_REQUIRE(seg >= kStaticMinSeg && seg <= 4096, "seg = %u: must be in [%u, 4096] (0 = 256)", seg, kStaticMinSeg);
+  SBWT_REQUIRE(rows <= kStaticRows && n_chars <= kStaticChars,
+               "batch of %llu rows / %llu characters exceeds the static scratch (at most %llu rows, %llu characters "
+               "per call)",
+               (unsigned long long)rows, (unsigned long long)n_chars, (unsigned long long)kStaticRows,
+               (unsigned long long)kStaticChars);
+  void* base = nullptr;
+  SBWT_TRY(scratch_on_device(&base));
+  u8* b = static_cast<u8*>(base);
+  sbwt_reads_plan v;  // a view of the static scratch: nothing here is freed
+  v.max_rows = kStaticRows;
+  v.max_chars = kStaticChars;
+  v.seg = seg;
+  v.d_packed = reinterpret_cast<u64*>(b);
+  b += kPackedBytes;
+  v.d_nmask = reinterpret_cast<u64*>(b);
+  b += kMaskBytes;
+  v.d_offsets = reinterpret_cast<u64*>(b);
+  b += kRowBytes;
+  v.d_work = reinterpret_cast<u64*>(b);
+  b += kRowBytes;
+  v.d_items = reinterpret_cast<u32*>(b);
+  b += kItemBytes;
+  v.d_scan_tmp = b;
+  v.scan_bytes = kScanReserve;
+  b += kScanReserve;
+  v.d_runs = reinterpret_cast<uint4*>(b);
+  v.run_cap = kRunCap;
+  b += kRunBytes;
+  v.d_run_count = reinterpret_cast<unsigned long long*>(b);
+  return plan_search_impl(&v, ix, d_chars, n_chars, d_row_starts, rows, d_out, d_offsets_out, flags, stream);
+}
