@@ -723,3 +723,26 @@ def test_enumerate_kmers_matches_oracle_builder():
     np.testing.assert_array_equal(km, want)
     # every enumerated k-mer is found, at its own node
     np.testing.assert_array_equal(sg.to_u64(ix.search_kmers(dev_u64(km))), np.flatnonzero(full).astype(np.uint64))
+
+
+@pytest.mark.parametrize("layout", [16, 32, 64])
+@pytest.mark.parametrize("m", [0, 8, 12])
+def test_counted_lines_equal_the_oracle_count(layout, m):
+    """The roofline's algorithmic bytes: the lines and steps the counted
+    k_search_kmers adds up equal orc_search_count's (pinned against brute
+    force) on the same k-mers, layout and depth."""
+    rng = np.random.default_rng(300 + layout + m)
+    g = genome(rng, 400_000)
+    k = 31
+    s = O.SBWT([g], k)
+    q = topk(sample_queries(rng, g, k, 20_000))
+    ix = sg.Index.from_rows(s.rows, s.n, k, group=s.group, config=sg.Config(presearch_depth=m, layout=layout))
+    oix = oracle_index(s, m)
+    cols = {16: 96, 32: 224, 64: 480}[layout]
+    want, steps, lines = oix.search_count(q, k, cols, layout)
+    ix.counters(reset=True)
+    got = sg.to_u64(ix.search_kmers(dev_u64(q), count=True))
+    ctr = ix.counters(reset=True)
+    np.testing.assert_array_equal(got, want)
+    assert ctr["lines"] == int(lines.sum()) and ctr["steps"] == int(steps.sum())
+    assert ctr["kmers"] == q.size and ctr["full_searches"] == q.size

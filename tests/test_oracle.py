@@ -270,3 +270,57 @@ def test_reference_cpu_path_one_mbp():
     for j in range(0, 5000, 97):
         assert s.label(int(found[j])) == pos_k[j]
     assert (base[5000:] == NONE).mean() > 0.99
+
+
+def _brute_lines(order, x, k, m, cpc, cbytes):
+    """Rank lines and steps of the reference search of x, from brute-force
+    suffix intervals (Python sort of the labels) instead of rank: a
+    presearch gather is one line; each narrowing step reads the lines of
+    columns l and r + 1 of the interval it narrows, and the search stops
+    after the step that empties the interval."""
+    n = len(order)
+    rev = [s[::-1] for s in order]
+    import bisect
+
+    def iv(i):
+        if i == 0:
+            return 0, n - 1
+        col = [r[:i] for r in rev]  # sorted: colex order agrees with every suffix length
+        p = x[:i][::-1]
+        return bisect.bisect_left(col, p), bisect.bisect_right(col, p) - 1
+
+    def line(c):
+        return (c // cpc) * cbytes // 128
+
+    lines, steps = (1 if m else 0), 0
+    for i in range(m, k):
+        l, r = iv(i)
+        lines += 1 if line(l) == line(r + 1) else 2
+        steps += 1
+        l2, r2 = iv(i + 1)
+        if l2 > r2:
+            break
+    return lines, steps
+
+
+@pytest.mark.parametrize("cpc,cbytes", [(96, 16), (224, 32), (480, 64), (5, 16)])
+@pytest.mark.parametrize("m", [0, 2, 4])
+def test_search_count_lines_against_brute_force(cpc, cbytes, m):
+    """orc_search_count's per-k-mer lines and steps (the algorithmic bytes of
+    the roofline) equal a count over brute-force intervals."""
+    rng = random.Random(cpc * 10 + m)
+    seqs = [rand_seq(rng, 2500), rand_seq(rng, 400)]
+    k = 7
+    order, _, _ = brute.sbwt(seqs, k)
+    s = O.SBWT(seqs, k)
+    assert s.n == len(order)
+    idx = O.Index(s.rows, s.n)
+    idx.presearch(m)
+    K = sorted(brute.kmer_set(seqs, k))
+    xs = [rng.choice(K) for _ in range(150)] + [rand_seq(rng, k) for _ in range(150)]
+    kmers = np.array([brute.encode_top(x) for x in xs], np.uint64)
+    ans, steps, lines = idx.search_count(kmers, k, cpc, cbytes)
+    for q, x in enumerate(xs):
+        bl, bs = _brute_lines(order, x, k, m, cpc, cbytes)
+        assert (int(lines[q]), int(steps[q])) == (bl, bs), (x, m)
+        assert int(ans[q]) == brute.search(order, set(K), x)
