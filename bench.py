@@ -75,7 +75,8 @@ def main() -> int:
     ap.add_argument("--no-roof", action="store_true", help="skip the random-gather roof measurement")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 3 times (HBM counter calibration) and exit")
-    ap.add_argument("--workload", choices=["large", "uniform", "pangenome", "reads", "batched", "prefilter", "text"],
+    ap.add_argument("--workload", choices=["large", "uniform", "pangenome", "reads", "batched", "prefilter", "text",
+                                           "small"],
                     default="large",
                     help="large (default): the 3 Gbp uniform-random index and the 20-haplotype pangenome index, "
                          "1B independent 31-mers each, 50%% present, plain / learned-filtered / Bloom-filtered "
@@ -83,13 +84,16 @@ def main() -> int:
                          "one of the two large indexes alone; reads: the 100 Mbp read search (1M x 150 bp); "
                          "batched: 10,000 uneven read sets with 5%% N, autotuned, in HIP-graph-captured "
                          "batches; prefilter: the learned negative prefilter (trained on the GPU) on a "
-                         "90%%-negative read mix, filtered vs plain; text: query text to result text")
+                         "90%%-negative read mix, filtered vs plain; text: query text to result text; small: the "
+                         "1 Mbp / 1M-query case, GPU beside the CPU oracle on all cores and on one")
     ap.add_argument("--no-reads", action="store_true", help="large: leave out the 100 Mbp read batch")
     ap.add_argument("--plain-only", action="store_true",
                     help="large indexes: the plain path only (no prefilter training or filtered paths), e.g. for a "
                          "kernel trace whose search dispatches are all timed ones")
     ap.add_argument("--shallow-depths", default="13,12",
                     help="large indexes: presearch depths also timed on the plain path (comma list; empty: none)")
+    ap.add_argument("--layout-sweep", default="32,64",
+                    help="uniform: also time these rank layouts (32 and 64 also with the cooperative kernel)")
     ap.add_argument("--negatives-like", choices=["uniform", "index"], default="index",
                     help="prefilter training negatives: uniform random k-mers, or random k-mers at the "
                          "index's base composition (each confirmed absent by the plain search)")
@@ -138,7 +142,7 @@ def main() -> int:
                   "with no exchange, so a multi-GPU figure is not produced by this build.", file=sys.stderr)
             return 2
         return {"large": run_large, "uniform": run_uniform, "batched": run_batched, "prefilter": run_prefilter,
-                "pangenome": run_pangenome, "text": run_text}[args.workload](args)
+                "pangenome": run_pangenome, "text": run_text, "small": run_small}[args.workload](args)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the index is replicated per GPU and queries shard by read set "
@@ -480,6 +484,45 @@ def large_index(kind: str, args) -> dict:
         log(f"{kind} m={m}: {Q / e / 1e9:.3f} G k-mers/s, kernel {km:.2f} ms, "
             f"{int(c['lines']) / Q:.2f} lines/k-mer; answers equal: {same}")
     res["shallower_presearch"] = depths
+    if depths:
+        ix.presearch(args.presearch)
+    # ---- other rank layouts, one lane or a group of lanes per k-mer ----------
+    layouts = {}
+    sweep = [int(x) for x in args.layout_sweep.split(",") if x] if kind == "uniform" else []
+    for lay in sweep:
+        t = time.time()
+        ix.set_layout(lay)
+        torch.cuda.synchronize()
+        relayout_s = time.time() - t
+        for coop in ([False, True] if lay >= 32 else [False]):
+            flags = _capi.SEARCH_COOPERATIVE if coop else 0
+
+            def run(ev, flags=flags):
+                if ev:
+                    ev[0].record(stream)
+                    ev[1].record(stream)
+                check(L.sbwt_search_kmers_flags(ix._h, _ptr(kmers), Q, _ptr(out), flags, s))
+                if ev:
+                    ev[2].record(stream)
+
+            sub = argparse.Namespace(**{**vars(args), "steps": max(3, min(args.steps, 5)), "warmup": 1})
+            e, _, km = _time_path(run, sub)
+            same = bool(torch.equal(out, ref))
+            ix.counters(reset=True)
+            check(L.sbwt_search_kmers_flags(ix._h, _ptr(kmers), Q, _ptr(out), flags | _capi.COUNT, s))
+            torch.cuda.synchronize()
+            c = ix.counters(reset=True)
+            name = f"L{lay}" + ("-cooperative" if coop else "")
+            layouts[name] = {"kmers_per_s": round(Q / e, 1), "kernel_ms": round(km, 3),
+                             "lines_per_kmer": round(int(c["lines"]) / Q, 3),
+                             "achieved_GBps": round(int(c["lines"]) * 128 / (km / 1e3) / 1e9, 1),
+                             "index_bytes": ix.info["device_bytes"], "relayout_s": round(relayout_s, 2),
+                             "answers_equal_plain": same}
+            log(f"{kind} {name}: {Q / e / 1e9:.3f} G k-mers/s, kernel {km:.2f} ms, "
+                f"{int(c['lines']) / Q:.2f} lines/k-mer; answers equal: {same}")
+    if sweep:
+        ix.set_layout(args.layout)
+        res["layouts"] = layouts
     if args.checkpoint and kind == "uniform":
         ix.presearch(args.presearch)
         res["checkpoint"] = image_round_trip(ix, kmers, out, ref.cpu().numpy().view(np.uint64))  # closes ix
@@ -506,7 +549,8 @@ def _filtered_paths(kind, ix, kmers, out, ref, maybe, paths, res, args):
     t = time.time()
     pf = train_prefilter(ix, positives=args.train_positives, negatives=args.train_negatives,
                          epochs=args.train_epochs, batch=args.train_batch, lr=args.lr, seed=7,
-                         threshold=args.threshold, fpr=0.01, negatives_like=args.negatives_like, log=log)
+                         threshold=args.threshold, fpr=0.01, negatives_like=args.negatives_like,
+                         calibrate_for=0.5, log=log)
     train_s = time.time() - t
     log(f"{kind}: prefilter ready in {train_s:.1f}s: {pf.stats}")
     t = time.time()
@@ -545,10 +589,90 @@ def _filtered_paths(kind, ix, kmers, out, ref, maybe, paths, res, args):
     res["bloom_only_vs_plain"] = round(paths["bloom_only"]["kmers_per_s"] / paths["plain"]["kmers_per_s"], 3)
     res["prefilter"] = {"train": pf.stats, "train_total_s": round(train_s, 1),
                         "learned_bloom_bytes": pf.bloom.bytes if pf.bloom else 0,
-                        "model_bytes": int(L.sbwt_mlp_param_count(pf.mlp._h)) * 2,
+                        "model_bytes": int(L.sbwt_mlp_param_count(pf.net._h)) * 2 if pf.mlp is not None else 0,
                         "bloom_only_bytes": bloom_all.bytes, "bloom_only_build_s": round(bloom_s, 1)}
     del pf, bloom_all
     torch.cuda.empty_cache()
+
+
+def small_case(args) -> dict:
+    """The CPU-runnable case: a seeded uniform 1,000,000 bp genome, k = 31,
+    presearch depth 8, 1,000,000 query 31-mers (half at uniform genome
+    positions, half uniform random, shuffled), searched independently with
+    the reference's semantics; the CPU oracle (poppy SB = 1024, the same
+    depth) runs all of them on every host core and on one, and its answers
+    are compared with the GPU's."""
+    import torch
+
+    import sbwt_gpu_amd as sg
+    from oracle import oracle as O
+    from sbwt_gpu_amd import _capi, synth
+    from sbwt_gpu_amd.api import _ptr, _stream
+
+    k, m, Q = 31, 8, 1_000_000
+    g = synth.genome_codes(1_000_000, seed=101, gc=0.5)
+    text = synth.to_ascii(g)
+    ix = sg.Index.build([text.tobytes()], k, sg.Config(presearch_depth=m))
+    rng = np.random.default_rng(102)
+    pos = rng.integers(0, g.size - k + 1, Q // 2)
+    present = np.zeros(pos.size, np.uint64)
+    for t in range(k):
+        present |= g[pos + t].astype(np.uint64) << np.uint64(62 - 2 * t)
+    q = np.concatenate([present, rng.integers(0, 2**62, Q - Q // 2, dtype=np.uint64) << np.uint64(2)])
+    rng.shuffle(q)
+    kmers = torch.from_numpy(q.view(np.int64)).cuda()
+    out = torch.empty_like(kmers)
+    L, s = _capi.load(), _stream()
+    stream = torch.cuda.current_stream()
+
+    def run(ev):
+        if ev:
+            ev[0].record(stream)
+            ev[1].record(stream)
+        _capi.check(L.sbwt_search_kmers(ix._h, _ptr(kmers), Q, _ptr(out), s))
+        if ev:
+            ev[2].record(stream)
+
+    sub = argparse.Namespace(steps=max(args.steps, 20), warmup=max(args.warmup, 3))
+    el, _, kern_ms = _time_path(run, sub)
+    got = out.cpu().numpy().view(np.uint64)
+    rows = ix.rows()
+    oix = O.Index(rows[:4], ix.n, sb=1024)
+    oix.presearch(m)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    oix.search(q[:10_000], k)  # warm
+    t = time.perf_counter()
+    want = oix.search(q, k)
+    cpu_all = Q / (time.perf_counter() - t)
+    t = time.perf_counter()
+    oix.search(q, k, threads=1)
+    cpu_one = Q / (time.perf_counter() - t)
+    parity = bool(np.array_equal(got, want))
+    ix.close()
+    log(f"1 Mbp case: GPU {Q / el / 1e9:.3f} G k-mers/s (kernel {kern_ms:.3f} ms), CPU oracle "
+        f"{cpu_all / 1e6:.2f} M/s on {threads} threads, {cpu_one / 1e6:.2f} M/s on 1; parity: {parity}")
+    return {"workload": "1 Mbp uniform genome, k=31, presearch 8, 1M query 31-mers (50% sampled, 50% random), "
+                        "independent searches",
+            "gpu_kmers_per_s": round(Q / el, 1), "gpu_ms_per_step": round(el * 1e3, 4),
+            "gpu_kernel_ms": round(kern_ms, 4),
+            "cpu_all_cores_kmers_per_s": round(cpu_all, 1), "cpu_cores": threads,
+            "cpu_one_core_kmers_per_s": round(cpu_one, 1), "cpu_kind": "port", "cpu_model": _cpu_model(),
+            "found_fraction": round(float((got != np.uint64(0xFFFFFFFFFFFFFFFF)).mean()), 4),
+            "gpu_equals_oracle": parity}
+
+
+def run_small(args) -> int:
+    res = small_case(args)
+    line = {"metric": METRIC, "value": res["gpu_kmers_per_s"], "unit": "k-mer queries/s", "n_gpus": 1,
+            "steps": max(args.steps, 20), "warmup": max(args.warmup, 3), "ms_per_step": res["gpu_ms_per_step"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+            "data": "synthetic (seeded)", "config": res, "roofline": None,
+            "cpu_baseline": {"value": res["cpu_all_cores_kmers_per_s"], "unit": "k-mer queries/s",
+                             "cores": res["cpu_cores"], "kind": "port", "sample": "all 1M queries",
+                             "single_thread_value": res["cpu_one_core_kmers_per_s"],
+                             "gpu_parity_on_sample": res["gpu_equals_oracle"]}}
+    print(json.dumps(line), flush=True)
+    return 0
 
 
 def _traffic_into(roofline: dict, traffic: dict | None, queries: int) -> None:
@@ -631,6 +755,7 @@ def run_large(args) -> int:
     line["roofline"], line["cpu_baseline"] = uni["roofline"], uni.get("cpu_baseline")
     line["pangenome_cpu_baseline"] = pan.get("cpu_baseline")
     line["reads_100Mbp_1Mx150bp"] = reads
+    line["small_1Mbp_1M_queries"] = None if args.no_cpu else small_case(args)
     print(json.dumps(line), flush=True)
     return 0
 
@@ -987,7 +1112,8 @@ def run_prefilter(args) -> int:
     t = time.time()
     pf = train_prefilter(ix, positives=args.train_positives, negatives=args.train_negatives,
                          epochs=args.train_epochs, batch=args.train_batch, lr=args.lr, seed=7,
-                         threshold=args.threshold, fpr=0.01, log=log)
+                         threshold=args.threshold, fpr=0.01, negatives_like=args.negatives_like,
+                         calibrate_for=args.negative_fraction, log=log)
     train_total_s = time.time() - t
     log(f"prefilter ready in {train_total_s:.1f}s: {pf.stats}")
     # the method without its learned part: a Bloom filter over every indexed k-mer
@@ -1001,8 +1127,8 @@ def run_prefilter(args) -> int:
     from sbwt_gpu_amd.learned_filter import random_kmers
     vneg = random_kmers(4_000_000, k, gen)
     vneg = vneg[ix.search_kmers(vneg) == -1]
-    zpos = pf.mlp.logits(allpos)
-    zneg = pf.mlp.logits(vneg)
+    zpos = pf.net.logits(allpos)
+    zneg = pf.net.logits(vneg)
     thr_table = []
     for thr in (-4.0, -2.0, -1.0, 0.0, 1.0, 2.0, 4.0):
         thr_table.append({"threshold": thr, "model_fn_fraction": round(float((zpos < thr).float().mean()), 4),
@@ -1050,12 +1176,20 @@ def run_prefilter(args) -> int:
         negatives) or "bloom" (Bloom over every indexed k-mer)."""
         def run(ev=None):
             prep(seg)
-            if filt == "learned":  # the matrix-core kernel takes extracted k-mers
+            if filt == "learned" and pf.mlp is not None:  # the matrix-core kernel takes extracted k-mers
                 check(L.sbwt_positions(_ptr(rstarts), _ptr(offsets), rows, k, _ptr(pos), s))
                 check(L.sbwt_extract_kmers(_ptr(packed), _ptr(pos), total, _ptr(kmers), s))
                 if ev:
                     ev[0].record(stream)
                 pf.maybe(kmers, out=maybe)
+            elif filt == "learned":  # calibrated without the network: its Bloom filter, from the reads
+                if ev:
+                    ev[0].record(stream)
+                if pf.bloom is not None:
+                    check(L.sbwt_bloom_query_reads(pf.bloom._h, _ptr(packed), _ptr(rstarts), _ptr(offsets), rows,
+                                                   _ptr(maybe), s))
+                else:
+                    maybe.fill_(1)
             elif filt == "bloom":  # straight from the packed reads
                 if ev:
                     ev[0].record(stream)
@@ -1146,10 +1280,14 @@ def run_prefilter(args) -> int:
     # (k_prefilter) timed alone, without the Bloom pass over its rejects
     fr = results["filtered"]
     mlp_ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    sg.prefilter(pf.mlp, pf.threshold, None, kmers, out=maybe)
+    # (the network's kernel is timed whether or not calibration kept it)
+    check(L.sbwt_positions(_ptr(rstarts), _ptr(offsets), rows, k, _ptr(pos), s))
+    check(L.sbwt_extract_kmers(_ptr(packed), _ptr(pos), total, _ptr(kmers), s))
+    net_thr = pf.threshold if pf.mlp is not None else 0.0
+    sg.prefilter(pf.net, net_thr, None, kmers, out=maybe)
     mlp_ev[0].record(stream)
     for _ in range(5):
-        sg.prefilter(pf.mlp, pf.threshold, None, kmers, out=maybe)
+        sg.prefilter(pf.net, net_thr, None, kmers, out=maybe)
     mlp_ev[1].record(stream)
     torch.cuda.synchronize()
     mlp_ms = mlp_ev[0].elapsed_time(mlp_ev[1]) / 5
@@ -1172,7 +1310,7 @@ def run_prefilter(args) -> int:
                    "kernel": ("k_search_reads" if fk else "k_reads_uniform") + " (masked)",
                    "kernel_avg_ms": fr["search_kernel_ms"],
                    "lines_per_launch": int(ctr["lines"])}
-    roofline = dict(mlp_roof if mlp_ms >= fr["search_kernel_ms"] else search_roof)
+    roofline = dict(mlp_roof if pf.mlp is not None and mlp_ms >= fr["search_kernel_ms"] else search_roof)
     roofline["other_kernel"] = search_roof if roofline["kernel"] == "k_prefilter" else mlp_roof
     cpu = None
     if not args.no_cpu:
@@ -1199,7 +1337,8 @@ def run_prefilter(args) -> int:
                                                   results["bloom_only"]["kmers_per_s"], 3),
                    "maybe_on_absent_kmers": round(m_on_absent, 4),
                    "maybe_on_negative_read_kmers": round(m_on_negreads, 4),
-                   "threshold": args.threshold, "threshold_table": thr_table,
+                   "threshold": pf.stats.get("threshold"), "threshold_table": thr_table,
+                   "network_used": pf.stats.get("network_used"),
                    "bloom_only_bytes": bloom_all.bytes, "bloom_only_zero_fn": bloom_all_zero_fn,
                    "train": pf.stats, "train_total_s": round(train_total_s, 1)},
         "roofline": roofline, "cpu_baseline": cpu,

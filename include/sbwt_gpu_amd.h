@@ -70,6 +70,13 @@ extern "C" {
                                      and never test a character. Ignored for
                                      k < 16, with SBWT_READS_N_AS_A, and for
                                      batches of 2^32 - 64 characters or more */
+#define SBWT_SEARCH_COOPERATIVE 128u /* sbwt_search_kmers_flags / _masked on
+                                     32- and 64-byte chunks: two or four lanes
+                                     per k-mer, each loading 16 bytes of every
+                                     rank chunk, ranks summed by shuffles
+                                     (include/gpu/rank_kernel.hpp:298-396); the
+                                     16-byte layout has one load per rank and
+                                     keeps one lane per k-mer */
 #define SBWT_COUNT 4u             /* counted variant: add the 128-byte lines the
                                      rank reads touch to the index's counters
                                      (sbwt_counters); also accepted by
@@ -180,7 +187,7 @@ int sbwt_presearch_get(const sbwt_index *ix, uint64_t *left, uint64_t *right);
  * include/gpu/device_stream.hpp:52-78). */
 int sbwt_counters(const sbwt_index *ix, uint64_t out[4], int reset, void *stream);
 
-/* sbwt_search_kmers with flags (0 or SBWT_COUNT). */
+/* sbwt_search_kmers with flags (0, SBWT_COUNT, SBWT_SEARCH_COOPERATIVE). */
 int sbwt_search_kmers_flags(const sbwt_index *ix, const uint64_t *d_kmers,
                             uint64_t count, uint64_t *d_out, uint32_t flags,
                             void *stream);

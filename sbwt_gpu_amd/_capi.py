@@ -19,6 +19,7 @@ READS_PERSISTENT = 8
 READS_NO_CERT = 16
 READS_SEGMENT_LANES = 32
 READS_RUN_ITEMS = 64
+SEARCH_COOPERATIVE = 128
 TEXT_LINES = 0
 TEXT_RECORDS = 1
 OUTPUT_TEXT = 0

@@ -191,12 +191,13 @@ class Index:
 
     # ---- search ---------------------------------------------------------------
     def search_kmers(self, kmers: torch.Tensor, out: torch.Tensor | None = None, count: bool = False,
-                     maybe: torch.Tensor | None = None) -> torch.Tensor:
+                     maybe: torch.Tensor | None = None, cooperative: bool = False) -> torch.Tensor:
         """kmers: device int64 tensor of top-aligned 2-bit k-mers; `maybe`
-        (uint8 per k-mer, e.g. from prefilter()) skips those marked 0."""
+        (uint8 per k-mer, e.g. from prefilter()) skips those marked 0;
+        cooperative: the wave-cooperative kernel (layouts 32 and 64)."""
         if out is None:
             out = torch.empty_like(kmers)
-        flags = _capi.COUNT if count else 0
+        flags = (_capi.COUNT if count else 0) | (_capi.SEARCH_COOPERATIVE if cooperative else 0)
         if maybe is None:
             check(self.lib.sbwt_search_kmers_flags(self._h, _ptr(kmers), kmers.numel(), _ptr(out), flags, _stream()))
         else:
@@ -559,8 +560,9 @@ class Bloom:
         check(self.lib.sbwt_bloom_get_words(self._h, C.c_void_p(w.ctypes.data), w.size))
         return w
 
-    def query(self, keys: torch.Tensor) -> torch.Tensor:
-        out = torch.empty(keys.numel(), dtype=torch.uint8, device="cuda")
+    def query(self, keys: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        if out is None:
+            out = torch.empty(keys.numel(), dtype=torch.uint8, device="cuda")
         check(self.lib.sbwt_bloom_query(self._h, _ptr(keys), keys.numel(), _ptr(out), _stream()))
         return out
 

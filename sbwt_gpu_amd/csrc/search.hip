@@ -162,6 +162,90 @@ __global__ __launch_bounds__(256) void k_search_kmers(DevIndex ix, const u64* __
   flush<COUNT>(ctr, t, q < count ? 1u : 0u);
 }
 
+// Wave-cooperative variant of k_search_kmers for chunks of 32 or 64 bytes:
+// a group of G = CB / 16 adjacent lanes searches one k-mer. Every rank
+// reads one 16-byte part of its chunk per lane (part p = lane mod G holds
+// the chunk's words 4p .. 4p+3; word 0 is the count before the chunk), each
+// lane counts the bits of its part below the rank's offset, and G - 1
+// shuffles sum the parts; every lane of the group then holds the same
+// interval and takes the same branch. So a rank is one 16-byte load per lane
+// instead of G loads by one lane, and the G loads of a group fall in one
+// chunk (the reference's basic-block popcount split over lanes with
+// shuffles, include/gpu/rank_kernel.hpp:298-396).
+template <int CB>
+__device__ __forceinline__ u32 part_rank(const uint4& v, u32 part, u32 o) {
+  const u32 w[4] = {v.x, v.y, v.z, v.w};
+  u32 r = 0;
+#pragma unroll
+  for (u32 i = 0; i < 4; i++) {
+    const u32 cw = 4 * part + i;  // chunk word
+    if (cw == 0) {
+      r += w[i];  // the count before the chunk
+    } else {
+      const u32 lo = 32 * (cw - 1);  // first column of payload word cw - 1
+      const u32 m = (o >= lo + 32) ? ~0u : ((o > lo) ? ((1u << (o - lo)) - 1u) : 0u);
+      r += __popc(w[i] & m);
+    }
+  }
+  return r;
+}
+
+template <int CB, bool COUNT>
+__global__ __launch_bounds__(256) void k_search_kmers_coop(DevIndex ix, const u64* __restrict__ kmers, u64 count,
+                                                           const u8* __restrict__ maybe, u64* __restrict__ out,
+                                                           unsigned long long* ctr) {
+  constexpr u32 G = CB / 16;
+  constexpr u32 COLS = Layout<CB>::COLS;
+  static_assert(G == 2 || G == 4, "two or four lanes per k-mer");
+  const u64 tid = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x;
+  const u64 q = tid / G;
+  const u32 part = static_cast<u32>(tid % G);
+  Tally t;
+  const bool live = q < count && !(maybe && !maybe[q]);
+  u64 res = kNone;
+  if (live) {
+    const u64 kmer = kmers[q];
+    u32 l, r, i;
+    if (ix.m) {
+      const uint2 e = ix.pre[kmer >> (64 - 2 * ix.m)];
+      l = e.x;
+      r = e.y;
+      i = ix.m;
+      if (COUNT && part == 0) t.lines++;
+    } else {
+      l = 0;
+      r = ix.C[4] - 1u;
+      i = 0;
+    }
+    if (COUNT && part == 0) t.full++;
+    for (; i < ix.k; i++) {
+      const u32 c = static_cast<u32>(kmer >> (62 - 2 * i)) & 3u;
+      const u32 a = l, b = r + 1u;
+      const u32 cha = a / COLS, chb = b / COLS;
+      if (COUNT && part == 0) {
+        t.lines += line_of<CB>(a) == line_of<CB>(b) ? 1u : 2u;
+        t.steps++;
+      }
+      const uint4* row = row_of(ix, c);
+      const uint4 va = row[static_cast<u64>(cha) * Layout<CB>::Q + part];
+      const uint4 vb = row[static_cast<u64>(chb) * Layout<CB>::Q + part];
+      u32 ra = part_rank<CB>(va, part, a - cha * COLS), rb = part_rank<CB>(vb, part, b - chb * COLS);
+#pragma unroll
+      for (u32 d = 1; d < G; d <<= 1) {
+        ra += __shfl_xor(ra, static_cast<int>(d), static_cast<int>(G));
+        rb += __shfl_xor(rb, static_cast<int>(d), static_cast<int>(G));
+      }
+      const u32 base = c_of(ix, c);
+      l = base + ra;
+      r = base + rb - 1u;
+      if (l > r) break;
+    }
+    res = l > r ? kNone : static_cast<u64>(l);
+  }
+  if (q < count && part == 0) out[q] = res;
+  flush<COUNT>(ctr, t, (q < count && part == 0) ? 1u : 0u);
+}
+
 template <int CB>
 __global__ __launch_bounds__(256) void k_search_positions(DevIndex ix, const u64* __restrict__ packed,
                                                           u64* __restrict__ pos_res, u64 count) {
@@ -347,24 +431,53 @@ __global__ __launch_bounds__(256) void k_search_reads(DevIndex ix, const u64* __
   flush<COUNT>(ctr, t, done);
 }
 
+// A dispatch holds fewer than 2^32 work-items, so a batch is searched in
+// pieces of at most kKmerPiece k-mers (four lanes each in the 64-byte
+// cooperative kernel).
+constexpr u64 kKmerPiece = 1ull << 29;
+
+template <int CB>
+static int launch_kmers_coop(const sbwt_index* ix, const u64* kmers, u64 count, const u8* maybe, u64* out,
+                             bool count_lines, hipStream_t s) {
+  const u32 bt = 256;
+  for (u64 at = 0; at < count; at += kKmerPiece) {
+    const u64 n = count - at < kKmerPiece ? count - at : kKmerPiece;
+    const u64 grid = (n * (CB / 16) + bt - 1) / bt;
+    const u8* mb = maybe ? maybe + at : nullptr;
+    if (count_lines)
+      k_search_kmers_coop<CB, true><<<grid, bt, 0, s>>>(dev_view(ix), kmers + at, n, mb, out + at, ix->d_counters);
+    else
+      k_search_kmers_coop<CB, false><<<grid, bt, 0, s>>>(dev_view(ix), kmers + at, n, mb, out + at, ix->d_counters);
+    SBWT_HIP(hipGetLastError());
+  }
+  return SBWT_OK;
+}
+
 template <int CB>
 static int launch_kmers(const sbwt_index* ix, const u64* kmers, u64 count, const u8* maybe, u64* out,
                         bool count_lines, hipStream_t s) {
   const u32 bt = 256;
-  const u64 grid = (count + bt - 1) / bt;
-  if (count_lines)
-    k_search_kmers<CB, true><<<grid, bt, 0, s>>>(dev_view(ix), kmers, count, maybe, out, ix->d_counters);
-  else
-    k_search_kmers<CB, false><<<grid, bt, 0, s>>>(dev_view(ix), kmers, count, maybe, out, ix->d_counters);
-  SBWT_HIP(hipGetLastError());
+  for (u64 at = 0; at < count; at += kKmerPiece) {
+    const u64 n = count - at < kKmerPiece ? count - at : kKmerPiece;
+    const u64 grid = (n + bt - 1) / bt;
+    const u8* mb = maybe ? maybe + at : nullptr;
+    if (count_lines)
+      k_search_kmers<CB, true><<<grid, bt, 0, s>>>(dev_view(ix), kmers + at, n, mb, out + at, ix->d_counters);
+    else
+      k_search_kmers<CB, false><<<grid, bt, 0, s>>>(dev_view(ix), kmers + at, n, mb, out + at, ix->d_counters);
+    SBWT_HIP(hipGetLastError());
+  }
   return SBWT_OK;
 }
 
 template <int CB>
 static int launch_positions(const sbwt_index* ix, const u64* packed, u64* pos, u64 count, hipStream_t s) {
   const u32 bt = 256;
-  k_search_positions<CB><<<(count + bt - 1) / bt, bt, 0, s>>>(dev_view(ix), packed, pos, count);
-  SBWT_HIP(hipGetLastError());
+  for (u64 at = 0; at < count; at += kKmerPiece) {
+    const u64 n = count - at < kKmerPiece ? count - at : kKmerPiece;
+    k_search_positions<CB><<<(n + bt - 1) / bt, bt, 0, s>>>(dev_view(ix), packed, pos + at, n);
+    SBWT_HIP(hipGetLastError());
+  }
   return SBWT_OK;
 }
 
@@ -441,11 +554,14 @@ static int search_kmers_impl(const sbwt_index* ix, const uint64_t* d_kmers, uint
   if (count == 0) return SBWT_OK;
   SBWT_REQUIRE(d_kmers && d_out, "device buffers must not be NULL");
   const bool cnt = (flags & SBWT_COUNT) != 0;
+  const bool coop = (flags & SBWT_SEARCH_COOPERATIVE) != 0;
   hipStream_t s = as_stream(stream);
   switch (ix->layout) {
     case 16: return launch_kmers<16>(ix, d_kmers, count, d_maybe, d_out, cnt, s);
-    case 32: return launch_kmers<32>(ix, d_kmers, count, d_maybe, d_out, cnt, s);
-    default: return launch_kmers<64>(ix, d_kmers, count, d_maybe, d_out, cnt, s);
+    case 32: return coop ? launch_kmers_coop<32>(ix, d_kmers, count, d_maybe, d_out, cnt, s)
+                         : launch_kmers<32>(ix, d_kmers, count, d_maybe, d_out, cnt, s);
+    default: return coop ? launch_kmers_coop<64>(ix, d_kmers, count, d_maybe, d_out, cnt, s)
+                         : launch_kmers<64>(ix, d_kmers, count, d_maybe, d_out, cnt, s);
   }
 }
 

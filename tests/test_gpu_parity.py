@@ -161,6 +161,55 @@ def test_random_rows_stress_rank():
             ix.close()
 
 
+@pytest.mark.parametrize("layout", [32, 64])
+def test_cooperative_kernel_matches_oracle(layout):
+    """SBWT_SEARCH_COOPERATIVE: two or four lanes per k-mer. Random rows drive
+    every chunk offset; an SBWT with half present queries, presearch depths 0
+    and 8, a mask, a count that is no multiple of the group, and the counted
+    variant (same lines as the one-lane kernel) must all answer alike."""
+    rng = np.random.default_rng(200 + layout)
+    n = 100_003
+    col = rng.integers(0, 4, n - 1)
+    perm = rng.permutation(n)[: n - 1]
+    rows = []
+    for c in range(4):
+        bits = np.zeros(((n + 4095) // 4096) * 4096, np.uint8)
+        bits[np.sort(perm[col == c])] = 1
+        rows.append(np.packbits(bits).view(">u8").astype(np.uint64))
+    q = rng.integers(0, 2**63, 50_001, dtype=np.uint64) * np.uint64(2)
+    for m in (0, 4):
+        oix = O.Index(rows, n)
+        oix.presearch(m)
+        ix = sg.Index.from_rows(rows, n, 12, config=sg.Config(presearch_depth=m, layout=layout))
+        np.testing.assert_array_equal(sg.to_u64(ix.search_kmers(dev_u64(q), cooperative=True)), oix.search(q, 12),
+                                      err_msg=f"random rows m={m}")
+        ix.close()
+    g = genome(rng, 200_000)
+    k = 31
+    s = O.SBWT([g], k)
+    q = topk(sample_queries(rng, g, k, 30_001))
+    for m in (0, 8):
+        want = oracle_index(s, m).search(q, k)
+        ix = sg.Index.from_rows(s.rows, s.n, k, group=s.group, config=sg.Config(presearch_depth=m, layout=layout))
+        dq = dev_u64(q)
+        got = sg.to_u64(ix.search_kmers(dq, cooperative=True))
+        np.testing.assert_array_equal(got, want, err_msg=f"m={m}")
+        assert (got != NONE).sum() >= 15_000
+        for cnt in (1, 2, 3, 5):  # groups cut by the end of the batch
+            np.testing.assert_array_equal(sg.to_u64(ix.search_kmers(dq[:cnt], cooperative=True)), want[:cnt])
+        maybe = torch.from_numpy((rng.random(q.size) < 0.7).astype(np.uint8)).cuda()
+        masked = sg.to_u64(ix.search_kmers(dq, maybe=maybe, cooperative=True))
+        np.testing.assert_array_equal(masked, np.where(maybe.cpu().numpy() == 1, want, NONE))
+        ix.counters(reset=True)
+        ix.search_kmers(dq, count=True)
+        lane = ix.counters(reset=True)
+        got_c = sg.to_u64(ix.search_kmers(dq, count=True, cooperative=True))
+        coop = ix.counters(reset=True)
+        np.testing.assert_array_equal(got_c, want)
+        assert coop == lane and lane["kmers"] == q.size
+        ix.close()
+
+
 def test_search_positions_reference_abi():
     rng = np.random.default_rng(3)
     g = genome(rng, 100_000)

@@ -360,3 +360,40 @@ def test_masked_kmer_search_equals_plain_where_maybe():
     got = sg.to_u64(ix.search_kmers(d, maybe=torch.from_numpy(mask).cuda()))
     np.testing.assert_array_equal(got, np.where(mask == 1, plain, NONE))
     assert (plain[3000:] != NONE).all()
+
+
+@pytest.mark.parametrize("case", ["uniform", "composition"])
+def test_calibrated_prefilter_keeps_answers(case):
+    """Calibration picks the cheapest of: the network at a quantile
+    threshold, the Bloom filter alone, or no filter, from held-out k-mers
+    and measured kernel costs. Whatever it picks keeps zero false negatives
+    on the index and the plain search's answers. On a uniform genome the
+    network is at chance (AUC near 0.5), so the network is not used; on a
+    GC-rich index it separates the composition (AUC well above 0.5)."""
+    rng = np.random.default_rng(40)
+    k = 31
+    if case == "uniform":
+        g = _genome(rng, 200_000)
+    else:
+        p = [0.14, 0.36, 0.36, 0.14]
+        g = "".join(ACGT[rng.choice(4, size=200_000, p=p)])
+    ix = sg.Index.build([g], k, sg.Config(presearch_depth=10))
+    pf = train_prefilter(ix, positives=150_000, negatives=150_000, batch=4096, epochs=2, lr=2e-3, seed=6,
+                         calibrate_for=0.9)
+    cal = pf.stats["calibration"]
+    assert cal is not None and cal["best"] in cal["options"]
+    assert min(o["cost"] for o in cal["options"]) == cal["best"]["cost"]
+    assert pf.stats["network_used"] == (pf.mlp is not None) == (cal["best"]["option"] == "network")
+    if case == "uniform":
+        assert abs(cal["auc"] - 0.5) < 0.05, cal["auc"]
+        assert not pf.stats["network_used"]
+    else:
+        assert cal["auc"] > 0.7, cal["auc"]
+    allk = ix.enumerate_kmers()
+    assert bool((pf.maybe(allk) == 1).all().item())
+    reads = _reads(rng, g, 100, 150) + _reads(rng, _genome(rng, 100_000), 400, 150)
+    batch = sg.ReadBatch.from_host(reads, k)
+    plain = sg.to_u64(ix.search_reads(batch))
+    km = batch.kmers()
+    np.testing.assert_array_equal(sg.to_u64(ix.search_reads(batch, maybe=pf.maybe(km))), plain)
+    np.testing.assert_array_equal(sg.to_u64(ix.search_kmers(km, maybe=pf.maybe(km))), sg.to_u64(ix.search_kmers(km)))
