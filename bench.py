@@ -10,8 +10,11 @@ through the reference's per-k-mer search semantics:
     indels of 1-10 bp (~0.25e9 columns).
 Each index runs three paths side by side on the same resident queries:
 plain, the learned prefilter (an MLP trained on the GPU plus a Bloom filter
-over its false negatives) before the masked search, and a Bloom filter over
-every indexed k-mer before the masked search; every path's answers must
+over its false negatives, or what its calibration keeps of it) before the
+masked search, and a Bloom filter over every indexed k-mer before the masked
+search. The indexes use 32-byte rank chunks searched by two lanes per k-mer
+(the cooperative kernel); the other layouts, one lane per k-mer and two or
+four, are timed beside it. Every path's answers must
 equal the plain ones bit for bit. The plain path is also timed with
 presearch tables of depth 13 and 12 beside the configured 17. `value` is the
 plain path on the 3 Gbp index. The 100 Mbp read batch (1M reads of 150 bp,
@@ -21,7 +24,9 @@ Prints one JSON line. `roofline` prices the search kernel's algorithmic
 bytes (128 B per rank line touched, counted by the kernel's counted variant
 on the same queries) against the 8 TB/s HBM peak and a measured
 random-line roof; `traffic` is FETCH_SIZE + WRITE_SIZE per launch from
-rocprofv3 --pmc child runs; `cpu_baseline` times the CPU oracle (a
+rocprofv3 --pmc child runs, each scaled by what it reads for a 1 GiB copy,
+with the L2's read requests by size (32, 64, 128 bytes) beside it as a
+check that needs no scale; `cpu_baseline` times the CPU oracle (a
 restatement of the reference's search: poppy rank, SB=1024) on a bounded
 sample of the same k-mers and checks that the GPU answers there are
 bit-identical.
@@ -92,8 +97,13 @@ def main() -> int:
                          "kernel trace whose search dispatches are all timed ones")
     ap.add_argument("--shallow-depths", default="13,12",
                     help="large indexes: presearch depths also timed on the plain path (comma list; empty: none)")
-    ap.add_argument("--layout-sweep", default="32,64",
-                    help="uniform: also time these rank layouts (32 and 64 also with the cooperative kernel)")
+    ap.add_argument("--large-layout", type=int, choices=[16, 32, 64], default=32,
+                    help="large / uniform / pangenome: rank layout of the index")
+    ap.add_argument("--kmer-kernel", choices=["lane", "cooperative"], default="cooperative",
+                    help="large / uniform / pangenome: one lane per k-mer, or a group of CB/16 lanes per k-mer "
+                         "(layouts 32 and 64)")
+    ap.add_argument("--layout-sweep", default="16,32,64",
+                    help="large: also time these rank layouts, one lane and (32, 64) cooperative per k-mer")
     ap.add_argument("--negatives-like", choices=["uniform", "index"], default="index",
                     help="prefilter training negatives: uniform random k-mers, or random k-mers at the "
                          "index's base composition (each confirmed absent by the plain search)")
@@ -262,6 +272,9 @@ def run_reads(args) -> dict:
         roofline["hbm_fetch_bytes_per_kmer"] = round(fetch["bytes"] / total, 2)
     if (args.traffic or {}).get("tcc"):
         roofline["tcc_hit_rate"] = args.traffic["tcc"]["hit_rate"]
+    rq = (args.traffic or {}).get("read_requests") or {}
+    if "read_bytes" in rq:
+        roofline["read_bytes_per_kmer_from_request_sizes"] = round(rq["read_bytes"] / total, 2)
     if not args.no_roof:
         roof = gather_roof()
         roofline["measured_random_roof"] = roof
@@ -375,7 +388,11 @@ def large_index(kind: str, args) -> dict:
     gen_s = time.time() - t
     log(f"{kind}: {G + R} bases generated on the GPU in {gen_s:.1f}s")
     t = time.time()
-    ix = sg.Index.build(chars[:G], k, sg.Config(presearch_depth=args.presearch, layout=args.layout), starts=starts)
+    ix = sg.Index.build(chars[:G], k, sg.Config(presearch_depth=args.presearch, layout=args.large_layout),
+                        starts=starts)
+    coop = args.kmer_kernel == "cooperative" and args.large_layout >= 32
+    kflags = _capi.SEARCH_COOPERATIVE if coop else 0
+    kname = "k_search_kmers_coop" if coop else "k_search_kmers"
     info = ix.info
     build_s = time.time() - t
     log(f"{kind}: index built on GPU in {build_s:.1f}s: n={info['n']} columns, "
@@ -415,7 +432,7 @@ def large_index(kind: str, args) -> dict:
         if ev:
             ev[0].record(stream)
             ev[1].record(stream)
-        check(L.sbwt_search_kmers(ix._h, _ptr(kmers), Q, _ptr(out), s))
+        check(L.sbwt_search_kmers_flags(ix._h, _ptr(kmers), Q, _ptr(out), kflags, s))
         if ev:
             ev[2].record(stream)
 
@@ -429,7 +446,7 @@ def large_index(kind: str, args) -> dict:
         f"found {found}/{Q}, sampled k-mers missing: {missing}")
     # algorithmic bytes: the counted variant on the same queries (untimed)
     ix.counters(reset=True)
-    ix.search_kmers(kmers, out=out, count=True)
+    check(L.sbwt_search_kmers_flags(ix._h, _ptr(kmers), Q, _ptr(out), kflags | _capi.COUNT, s))
     torch.cuda.synchronize()
     ctr = ix.counters(reset=True)
     counted_same = bool(torch.equal(out, ref))
@@ -437,13 +454,14 @@ def large_index(kind: str, args) -> dict:
     kern_s = kern_ms / 1e3
     achieved = lines * 128 / kern_s / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None, "kernel": "k_search_kmers",
+                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None, "kernel": kname,
                 "kernel_avg_ms": round(kern_ms, 3), "lines_per_launch": lines,
                 "bytes_per_kmer": round(lines * 128 / Q, 2), "steps_per_launch": int(ctr["steps"]),
                 "floor_bytes_per_kmer": round(128 * int(ctr["steps"]) / Q, 2),
                 "index_bytes": info["device_bytes"], "counted_equals_timed": counted_same}
     res = {"workload": desc, "bases": G, "index_columns": info["n"], "index_build_s": round(build_s, 1),
-           "generate_s": round(gen_s, 1), "layout": f"L{info['layout']}", "presearch": info["presearch_depth"],
+           "generate_s": round(gen_s, 1), "layout": f"L{info['layout']}", "kmer_kernel": kname,
+           "presearch": info["presearch_depth"],
            "queries": Q, "found_fraction": round(found / Q, 4), "sampled_kmers_all_found": missing == 0,
            "roofline": roofline}
 
@@ -452,7 +470,7 @@ def large_index(kind: str, args) -> dict:
                        "answers_equal_plain": True}}
     res["paths"] = paths
     if not args.plain_only:
-        _filtered_paths(kind, ix, kmers, out, ref, maybe, paths, res, args)
+        _filtered_paths(kind, ix, kmers, out, ref, maybe, paths, res, args, kflags)
     torch.cuda.empty_cache()
 
     # ---- the CPU baseline on a sample, with GPU parity ----------------------
@@ -473,7 +491,7 @@ def large_index(kind: str, args) -> dict:
         e, _, km = _time_path(plain, sub)
         same = bool(torch.equal(out, ref))
         ix.counters(reset=True)
-        ix.search_kmers(kmers, out=out, count=True)
+        check(L.sbwt_search_kmers_flags(ix._h, _ptr(kmers), Q, _ptr(out), kflags | _capi.COUNT, s))
         torch.cuda.synchronize()
         c = ix.counters(reset=True)
         depths[str(m)] = {"kmers_per_s": round(Q / e, 1), "kernel_ms": round(km, 3),
@@ -488,14 +506,16 @@ def large_index(kind: str, args) -> dict:
         ix.presearch(args.presearch)
     # ---- other rank layouts, one lane or a group of lanes per k-mer ----------
     layouts = {}
-    sweep = [int(x) for x in args.layout_sweep.split(",") if x] if kind == "uniform" else []
+    sweep = [int(x) for x in args.layout_sweep.split(",") if x]
     for lay in sweep:
         t = time.time()
         ix.set_layout(lay)
         torch.cuda.synchronize()
         relayout_s = time.time() - t
-        for coop in ([False, True] if lay >= 32 else [False]):
-            flags = _capi.SEARCH_COOPERATIVE if coop else 0
+        for coop_ in ([False, True] if lay >= 32 else [False]):
+            if lay == args.large_layout and coop_ == coop:
+                continue  # the configured path, timed above
+            flags = _capi.SEARCH_COOPERATIVE if coop_ else 0
 
             def run(ev, flags=flags):
                 if ev:
@@ -512,7 +532,7 @@ def large_index(kind: str, args) -> dict:
             check(L.sbwt_search_kmers_flags(ix._h, _ptr(kmers), Q, _ptr(out), flags | _capi.COUNT, s))
             torch.cuda.synchronize()
             c = ix.counters(reset=True)
-            name = f"L{lay}" + ("-cooperative" if coop else "")
+            name = f"L{lay}" + ("-cooperative" if coop_ else "")
             layouts[name] = {"kmers_per_s": round(Q / e, 1), "kernel_ms": round(km, 3),
                              "lines_per_kmer": round(int(c["lines"]) / Q, 3),
                              "achieved_GBps": round(int(c["lines"]) * 128 / (km / 1e3) / 1e9, 1),
@@ -521,7 +541,7 @@ def large_index(kind: str, args) -> dict:
             log(f"{kind} {name}: {Q / e / 1e9:.3f} G k-mers/s, kernel {km:.2f} ms, "
                 f"{int(c['lines']) / Q:.2f} lines/k-mer; answers equal: {same}")
     if sweep:
-        ix.set_layout(args.layout)
+        ix.set_layout(args.large_layout)
         res["layouts"] = layouts
     if args.checkpoint and kind == "uniform":
         ix.presearch(args.presearch)
@@ -532,7 +552,7 @@ def large_index(kind: str, args) -> dict:
     return res
 
 
-def _filtered_paths(kind, ix, kmers, out, ref, maybe, paths, res, args):
+def _filtered_paths(kind, ix, kmers, out, ref, maybe, paths, res, args, kflags=0):
     """The learned prefilter and the Bloom filter over every indexed k-mer,
     each before the masked search, timed on the same queries as the plain
     path; their answers are compared with the plain ones (ref) on the device."""
@@ -571,7 +591,7 @@ def _filtered_paths(kind, ix, kmers, out, ref, maybe, paths, res, args):
                 check(L.sbwt_bloom_query(bloom_all._h, _ptr(kmers), Q, _ptr(maybe), s))
             if ev:
                 ev[1].record(stream)
-            check(L.sbwt_search_kmers_masked(ix._h, _ptr(kmers), Q, _ptr(maybe), _ptr(out), 0, s))
+            check(L.sbwt_search_kmers_masked(ix._h, _ptr(kmers), Q, _ptr(maybe), _ptr(out), kflags, s))
             if ev:
                 ev[2].record(stream)
         return run
@@ -687,6 +707,9 @@ def _traffic_into(roofline: dict, traffic: dict | None, queries: int) -> None:
         roofline["hbm_fetch_bytes_per_kmer"] = round(fetch["bytes"] / traffic["queries_per_launch"], 2)
     if (traffic or {}).get("tcc"):
         roofline["tcc_hit_rate"] = traffic["tcc"]["hit_rate"]
+    rq = (traffic or {}).get("read_requests") or {}
+    if "read_bytes" in rq:
+        roofline["read_bytes_per_kmer_from_request_sizes"] = round(rq["read_bytes"] / traffic["queries_per_launch"], 2)
 
 
 def _large_line(res: dict, args) -> dict:
@@ -1409,7 +1432,9 @@ def measure_traffic(args):
     times): FETCH_SIZE, then WRITE_SIZE (KiB per dispatch), each scaled by
     the factor that makes it read 1 GiB for that copy (HIP's device-to-device
     copy kernel, __amd_rocclr_copyBuffer); a third run reads TCC_HIT_sum and
-    TCC_MISS_sum, the L2 hit rate of the search's requests."""
+    TCC_MISS_sum, the L2 hit rate of the search's requests; a fourth the
+    L2's read requests to memory by size (TCC_EA0_RDREQ_32B/64B/128B), whose
+    bytes need no scale."""
     import csv
     import shutil
     import subprocess
@@ -1419,17 +1444,22 @@ def measure_traffic(args):
     if not prof:
         return {"error": "rocprofv3 not found"}
     large = args.workload in ("large", "uniform")
-    kern = "k_search_kmers" if large else (
+    kern = ("k_search_kmers_coop<" if args.kmer_kernel == "cooperative" and args.large_layout >= 32
+            else "k_search_kmers<") if large else (
         "k_reads_uniform" if args.mode in ("default", "persistent", "persistent-nocert", "persistent-independent") else "k_search_reads")
     child = [sys.executable, "-u", os.path.abspath(__file__), "--pmc-child",
              "--workload", "uniform" if large else args.workload,
              "--mode", args.mode, "--layout", str(args.layout), "--presearch", str(args.presearch),
+             "--large-layout", str(args.large_layout), "--kmer-kernel", args.kmer_kernel,
              "--seg", str(args.seg), "--genome-bp", str(args.genome_bp), "--reads", str(args.reads),
              "--read-len", str(args.read_len), "--k", str(args.k), "--large-bp", str(args.large_bp),
              "--queries", str(min(args.queries, 250_000_000))]
     out = {"kernel": kern, "counters": {}}
     env = dict(os.environ, TMPDIR=tempfile.gettempdir())
-    for ctrs in (("FETCH_SIZE",), ("WRITE_SIZE",), ("TCC_HIT_sum", "TCC_MISS_sum")):
+
+    def run_pass(ctrs):
+        """Per-dispatch values of ctrs for the copies (in dispatch order) and
+        the timed search kernel; raises RuntimeError with what went wrong."""
         tag = ctrs[0]
         d = tempfile.mkdtemp(prefix=f"sbwt_pmc_{tag}_")
         cmd = [prof, "--pmc", *ctrs, "-f", "csv", "-d", d, "-o", "run", "--", *child]
@@ -1437,10 +1467,10 @@ def measure_traffic(args):
         try:
             r = subprocess.run(cmd, capture_output=True, text=True, timeout=300, env=env, cwd=tempfile.gettempdir())
         except subprocess.TimeoutExpired:
-            return {"error": f"{tag} run timed out"}
+            raise RuntimeError(f"{tag} run timed out")
         files = [os.path.join(dp, f) for dp, _, fs in os.walk(d) for f in fs if f.endswith("counter_collection.csv")]
         if not files:
-            return {"error": f"{tag} run produced no counters (rc {r.returncode}): {r.stderr[-300:]}"}
+            raise RuntimeError(f"{tag} run produced no counters (rc {r.returncode}): {r.stderr[-300:]}")
         copy, search = {c: [] for c in ctrs}, {c: [] for c in ctrs}
         with open(files[0]) as fh:
             rows_ = list(csv.DictReader(fh))
@@ -1456,10 +1486,17 @@ def measure_traffic(args):
                 search[c].append(v)
         shutil.rmtree(d, ignore_errors=True)
         if not all(search.values()):
-            return {"error": f"{tag}: search kernel missing from the counters"}
+            raise RuntimeError(f"{tag}: search kernel missing from the counters")
         # the median dispatch: the build's own check also launches the search
         # kernel, on a few million sampled k-mers
-        med = {c: float(np.median(v)) for c, v in search.items()}
+        return copy, {c: float(np.median(v)) for c, v in search.items()}
+
+    for ctrs in (("FETCH_SIZE",), ("WRITE_SIZE",), ("TCC_HIT_sum", "TCC_MISS_sum")):
+        tag = ctrs[0]
+        try:
+            copy, med = run_pass(ctrs)
+        except RuntimeError as e:
+            return {"error": str(e)}
         if len(ctrs) == 2:  # the L2 hit rate: a ratio, no calibration
             hit, miss = med[ctrs[0]], med[ctrs[1]]
             out["tcc"] = {"hit_per_dispatch": round(hit), "miss_per_dispatch": round(miss),
@@ -1478,6 +1515,27 @@ def measure_traffic(args):
         kib = med[tag]
         out["counters"][tag] = {"search_KiB_per_dispatch": round(kib, 1), "copy_KiB": round(copy_kib, 1),
                                 "scale": round(factor, 4), "bytes": round(kib * 1024 * factor)}
+    # The L2's read requests to memory by size. FETCH_SIZE counts a request as
+    # 64 bytes unless TCC_BUBBLE marks it 128 (on gfx950 TCC_BUBBLE reads 0),
+    # so 128-byte requests count half: the copy's x2 factor. Read bytes from
+    # the request sizes need no factor; the copies check them (1 GiB each).
+    sizes = ("TCC_EA0_RDREQ_sum", "TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_128B_sum")
+
+    def req_bytes(m):
+        tot, r32, r64, r128 = (m[c] for c in sizes)
+        return 128 * r128 + 64 * r64 + 32 * r32 + 64 * max(tot - r32 - r64 - r128, 0.0)
+
+    try:
+        copy, med = run_pass(sizes)
+        big = [dict(zip(sizes, v)) for v in zip(*(copy[c][:3] for c in sizes))]
+        copy_b = float(np.mean([req_bytes(m) for m in big])) if big else 0.0
+        out["read_requests"] = {
+            "per_dispatch": {c.replace("TCC_EA0_RDREQ", "").strip("_").replace("_sum", "").replace("sum", "all"):
+                             round(med[c]) for c in sizes},
+            "read_bytes": round(req_bytes(med)),
+            "copy_read_bytes_over_1GiB": round(copy_b / (1 << 30), 4) if copy_b else None}
+    except RuntimeError as e:
+        out["read_requests"] = {"error": str(e)}
     out["queries_per_launch"] = min(args.queries, 250_000_000) if large else None
     return out
 
