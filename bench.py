@@ -12,9 +12,10 @@ Each index runs three paths side by side on the same resident queries:
 plain, the learned prefilter (an MLP trained on the GPU plus a Bloom filter
 over its false negatives, or what its calibration keeps of it) before the
 masked search, and a Bloom filter over every indexed k-mer before the masked
-search. The indexes use 32-byte rank chunks searched by two lanes per k-mer
-(the cooperative kernel); the other layouts, one lane per k-mer and two or
-four, are timed beside it. Every path's answers must
+search. The uniform index uses 32-byte rank chunks searched by two lanes per
+k-mer (the cooperative kernel), the pangenome 64-byte chunks searched by one
+lane per k-mer (the fastest of each index's layout sweep); the other layouts,
+one lane per k-mer and two or four, are timed beside them. Every path's answers must
 equal the plain ones bit for bit. The plain path is also timed with
 presearch tables of depth 13 and 12 beside the configured 17. `value` is the
 plain path on the 3 Gbp index. The 100 Mbp read batch (1M reads of 150 bp,
@@ -34,6 +35,7 @@ bit-identical.
 from __future__ import annotations
 
 import argparse
+import re
 import ctypes as C
 import json
 import os
@@ -97,10 +99,13 @@ def main() -> int:
                          "kernel trace whose search dispatches are all timed ones")
     ap.add_argument("--shallow-depths", default="13,12",
                     help="large indexes: presearch depths also timed on the plain path (comma list; empty: none)")
-    ap.add_argument("--large-layout", type=int, choices=[16, 32, 64], default=32,
-                    help="large / uniform / pangenome: rank layout of the index")
+    ap.add_argument("--large-layout", choices=["default", "auto", "16", "32", "64"], default="default",
+                    help="large / uniform / pangenome: rank layout of the index; default: 32-byte chunks with "
+                         "the cooperative kernel for the uniform index, 64-byte chunks one lane per k-mer for the "
+                         "pangenome (the fastest of the layout sweep on each); auto: what the k-mer autotuner "
+                         "times fastest on 64M of the queries")
     ap.add_argument("--kmer-kernel", choices=["lane", "cooperative"], default="cooperative",
-                    help="large / uniform / pangenome: one lane per k-mer, or a group of CB/16 lanes per k-mer "
+                    help="with a given --large-layout: one lane per k-mer, or a group of CB/16 lanes per k-mer "
                          "(layouts 32 and 64)")
     ap.add_argument("--layout-sweep", default="16,32,64",
                     help="large: also time these rank layouts, one lane and (32, 64) cooperative per k-mer")
@@ -388,11 +393,13 @@ def large_index(kind: str, args) -> dict:
     gen_s = time.time() - t
     log(f"{kind}: {G + R} bases generated on the GPU in {gen_s:.1f}s")
     t = time.time()
-    ix = sg.Index.build(chars[:G], k, sg.Config(presearch_depth=args.presearch, layout=args.large_layout),
+    # the default per index: the fastest of the layout sweep over several runs
+    # (the others stay within 2-8%; the autotuner's pick varies among those
+    # within 2% from run to run)
+    default = {"uniform": (32, True), "pangenome": (64, False)}[kind]
+    build_layout = {"default": default[0], "auto": 32}.get(args.large_layout) or int(args.large_layout)
+    ix = sg.Index.build(chars[:G], k, sg.Config(presearch_depth=args.presearch, layout=build_layout),
                         starts=starts)
-    coop = args.kmer_kernel == "cooperative" and args.large_layout >= 32
-    kflags = _capi.SEARCH_COOPERATIVE if coop else 0
-    kname = "k_search_kmers_coop" if coop else "k_search_kmers"
     info = ix.info
     build_s = time.time() - t
     log(f"{kind}: index built on GPU in {build_s:.1f}s: n={info['n']} columns, "
@@ -426,6 +433,19 @@ def large_index(kind: str, args) -> dict:
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
     log(f"{kind}: {Q} queries ready")
+    tune = None
+    if args.large_layout == "auto":
+        from sbwt_gpu_amd.autotune import autotune_kmers
+        tune = autotune_kmers(ix, kmers, log=log)
+        layout, coop = tune["best"]["layout"], tune["best"]["kernel"] == "cooperative"
+    elif args.large_layout == "default":
+        layout, coop = default
+    else:
+        layout, coop = build_layout, args.kmer_kernel == "cooperative" and build_layout >= 32
+    kflags = _capi.SEARCH_COOPERATIVE if coop else 0
+    kname = "k_search_kmers_coop" if coop else "k_search_kmers"
+    info = ix.info
+    log(f"{kind}: kmer kernel: {kname}<{layout},")  # (read by the counter runs' parent)
     stream = torch.cuda.current_stream()
 
     def plain(ev):
@@ -462,6 +482,7 @@ def large_index(kind: str, args) -> dict:
     res = {"workload": desc, "bases": G, "index_columns": info["n"], "index_build_s": round(build_s, 1),
            "generate_s": round(gen_s, 1), "layout": f"L{info['layout']}", "kmer_kernel": kname,
            "presearch": info["presearch_depth"],
+           "kmer_autotune": tune,
            "queries": Q, "found_fraction": round(found / Q, 4), "sampled_kmers_all_found": missing == 0,
            "roofline": roofline}
 
@@ -513,7 +534,7 @@ def large_index(kind: str, args) -> dict:
         torch.cuda.synchronize()
         relayout_s = time.time() - t
         for coop_ in ([False, True] if lay >= 32 else [False]):
-            if lay == args.large_layout and coop_ == coop:
+            if lay == layout and coop_ == coop:
                 continue  # the configured path, timed above
             flags = _capi.SEARCH_COOPERATIVE if coop_ else 0
 
@@ -541,7 +562,7 @@ def large_index(kind: str, args) -> dict:
             log(f"{kind} {name}: {Q / e / 1e9:.3f} G k-mers/s, kernel {km:.2f} ms, "
                 f"{int(c['lines']) / Q:.2f} lines/k-mer; answers equal: {same}")
     if sweep:
-        ix.set_layout(args.large_layout)
+        ix.set_layout(layout)
         res["layouts"] = layouts
     if args.checkpoint and kind == "uniform":
         ix.presearch(args.presearch)
@@ -1444,8 +1465,9 @@ def measure_traffic(args):
     if not prof:
         return {"error": "rocprofv3 not found"}
     large = args.workload in ("large", "uniform")
-    kern = ("k_search_kmers_coop<" if args.kmer_kernel == "cooperative" and args.large_layout >= 32
-            else "k_search_kmers<") if large else (
+    # large: the child names the k-mer kernel it times (its autotuner may pick
+    # any layout and kernel) and its timed dispatch is that kernel's last
+    kern = None if large else (
         "k_reads_uniform" if args.mode in ("default", "persistent", "persistent-nocert", "persistent-independent") else "k_search_reads")
     child = [sys.executable, "-u", os.path.abspath(__file__), "--pmc-child",
              "--workload", "uniform" if large else args.workload,
@@ -1471,6 +1493,13 @@ def measure_traffic(args):
         files = [os.path.join(dp, f) for dp, _, fs in os.walk(d) for f in fs if f.endswith("counter_collection.csv")]
         if not files:
             raise RuntimeError(f"{tag} run produced no counters (rc {r.returncode}): {r.stderr[-300:]}")
+        name_ = kern
+        if large:
+            m = re.search(r"kmer kernel: (\S+)", r.stderr)
+            if not m:
+                raise RuntimeError(f"{tag}: the child did not name its k-mer kernel")
+            name_ = m.group(1)
+            out["kernel"] = name_
         copy, search = {c: [] for c in ctrs}, {c: [] for c in ctrs}
         with open(files[0]) as fh:
             rows_ = list(csv.DictReader(fh))
@@ -1482,14 +1511,15 @@ def measure_traffic(args):
                 continue
             if "__amd_rocclr_copyBuffer" in name:
                 copy[c].append(v)
-            elif kern in name and ", true>" not in name.split("(")[0][-8:]:
+            elif name_ in name and ", true>" not in name.split("(")[0][-8:]:
                 search[c].append(v)
         shutil.rmtree(d, ignore_errors=True)
         if not all(search.values()):
             raise RuntimeError(f"{tag}: search kernel missing from the counters")
-        # the median dispatch: the build's own check also launches the search
-        # kernel, on a few million sampled k-mers
-        return copy, {c: float(np.median(v)) for c, v in search.items()}
+        # reads: the median dispatch (the build's own check also launches the
+        # search kernel, on a few million sampled k-mers); large: the last
+        # (the autotuner's trials run before it, on fewer k-mers)
+        return copy, {c: float(v[-1] if large else np.median(v)) for c, v in search.items()}
 
     for ctrs in (("FETCH_SIZE",), ("WRITE_SIZE",), ("TCC_HIT_sum", "TCC_MISS_sum")):
         tag = ctrs[0]

@@ -80,3 +80,48 @@ def autotune(index: Index, chars: torch.Tensor, row_starts: torch.Tensor, *, lay
     if index.info["presearch_depth"] != best["presearch_depth"]:
         index.presearch(best["presearch_depth"])
     return {"best": best, "trials": trials}
+
+
+KMER_KERNELS = ("lane", "cooperative")
+
+
+def autotune_kmers(index: Index, kmers: torch.Tensor, *, layouts=(16, 32, 64), sample: int = 1 << 26,
+                   reps: int = 3, log=None) -> dict:
+    """The k-mer search (sbwt_search_kmers_flags) of the first `sample`
+    top-aligned k-mers under every rank layout, one lane per k-mer or the
+    cooperative kernel (layouts 32 and 64); checks that all give the same
+    answers and leaves the index in the fastest layout. Which wins depends
+    on the index: on a 3 Gbp uniform index two lanes per 32-byte chunk, on a
+    pangenome whose rows fit the last-level cache one lane per 64-byte
+    chunk. Returns {"best": {"layout", "kernel", "ms"}, "trials": [...]}."""
+    q = kmers[: min(sample, kmers.numel())]
+    out = torch.empty_like(q)
+    stream = torch.cuda.current_stream()
+    ref = None
+    trials = []
+    for layout in layouts:
+        if index.info["layout"] != layout:
+            index.set_layout(layout)
+        for kern in (KMER_KERNELS if layout >= 32 else ("lane",)):
+            coop = kern == "cooperative"
+            index.search_kmers(q, out=out, cooperative=coop)  # warm
+            times = []
+            for _ in range(reps):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                index.search_kmers(q, out=out, cooperative=coop)
+                e1.record(stream)
+                e1.synchronize()
+                times.append(e0.elapsed_time(e1))
+            if ref is None:
+                ref = out.clone()
+            elif not torch.equal(out, ref):
+                raise RuntimeError(f"autotune_kmers: answers differ for layout={layout} kernel={kern}")
+            ms = sorted(times)[len(times) // 2]
+            trials.append({"layout": layout, "kernel": kern, "ms": ms})
+            if log:
+                log(f"autotune_kmers layout=L{layout} {kern}: {ms:.3f} ms for {q.numel()} k-mers")
+    best = min(trials, key=lambda t: t["ms"])
+    if index.info["layout"] != best["layout"]:
+        index.set_layout(best["layout"])
+    return {"best": best, "trials": trials}

@@ -432,16 +432,17 @@ __global__ __launch_bounds__(256) void k_search_reads(DevIndex ix, const u64* __
 }
 
 // A dispatch holds fewer than 2^32 work-items, so a batch is searched in
-// pieces of at most kKmerPiece k-mers (four lanes each in the 64-byte
-// cooperative kernel).
-constexpr u64 kKmerPiece = 1ull << 29;
+// pieces of at most kKmerPiece work-items: kKmerPiece / G k-mers for G lanes
+// per k-mer (a billion k-mers is one dispatch in every kernel).
+constexpr u64 kKmerPiece = (1ull << 32) - (1ull << 16);
 
 template <int CB>
 static int launch_kmers_coop(const sbwt_index* ix, const u64* kmers, u64 count, const u8* maybe, u64* out,
                              bool count_lines, hipStream_t s) {
   const u32 bt = 256;
-  for (u64 at = 0; at < count; at += kKmerPiece) {
-    const u64 n = count - at < kKmerPiece ? count - at : kKmerPiece;
+  constexpr u64 piece = kKmerPiece / (CB / 16);
+  for (u64 at = 0; at < count; at += piece) {
+    const u64 n = count - at < piece ? count - at : piece;
     const u64 grid = (n * (CB / 16) + bt - 1) / bt;
     const u8* mb = maybe ? maybe + at : nullptr;
     if (count_lines)

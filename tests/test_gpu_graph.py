@@ -98,6 +98,24 @@ def test_autotune_keeps_answers_and_applies_best():
     np.testing.assert_array_equal(sg.to_u64(out), want)
 
 
+def test_autotune_kmers_keeps_answers_and_applies_best():
+    rng, s, ix, oix, chars, starts, set_starts = _setup(seed=4)
+    g = synth.genome_codes(150_000, 4, gc=0.45)
+    pos = rng.integers(0, g.size - 31 + 1, 20_000)
+    q = np.zeros(pos.size, np.uint64)
+    for t in range(31):
+        q |= g[pos + t].astype(np.uint64) << np.uint64(62 - 2 * t)
+    q = np.concatenate([q, rng.integers(0, 2**62, 20_000, dtype=np.uint64) << np.uint64(2)])
+    want = oix.search(q, 31)
+    dq = torch.from_numpy(q.view(np.int64)).cuda()
+    res = autotune.autotune_kmers(ix, dq, sample=30_000, reps=1)
+    best = res["best"]
+    assert len(res["trials"]) == 5 and ix.info["layout"] == best["layout"]
+    got = sg.to_u64(ix.search_kmers(dq, cooperative=best["kernel"] == "cooperative"))
+    np.testing.assert_array_equal(got, want)
+    assert (want != sg.NONE).sum() >= 20_000
+
+
 # ---------------------------------------------------------------------------
 # run items (SBWT_READS_RUN_ITEMS): work items are the rows' N-free runs
 
