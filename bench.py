@@ -1316,6 +1316,9 @@ def run_prefilter(args) -> int:
     per_read = args.read_len - k + 1
     pos_kmer = np.repeat(is_pos_read, per_read)
     neg_frac_kmers = float(1.0 - (plain != none).mean())
+    # the batch's k-mers (the paths extract them only when the network runs)
+    check(L.sbwt_positions(_ptr(rstarts), _ptr(offsets), rows, k, _ptr(pos), s))
+    check(L.sbwt_extract_kmers(_ptr(packed), _ptr(pos), total, _ptr(kmers), s))
     maybe_l = pf.maybe(kmers).cpu().numpy().astype(bool)
     m_on_absent = float(maybe_l[plain == none].mean())
     m_on_negreads = float(maybe_l[~pos_kmer].mean())
