@@ -269,22 +269,34 @@ struct Walker {
     start_search(ix, rd, t, shrt ? P_CERTS : P_CERTP, d, cap < ix.k ? cap : ix.k);
   }
 
-  // does the k-mer at j contain a non-ACGT character?
+  // does the k-mer at j contain a non-ACGT character? nN = the first
+  // non-ACGT position at or after j among the characters the item's k-mers
+  // hold (up to jend + k - 1), or that bound when there is none. The scan
+  // loads four mask words at a time before testing any (one wait per 256
+  // characters instead of one per word: a 150-character read is one round
+  // trip), and stops at the item's end, not the row's.
   __device__ __forceinline__ bool has_n(const ReadsDev& rd, u32 k) {
     if (nN == kNoPos || nN < j) {
-      // next non-ACGT at or after j, within the row (rowk + k - 1 characters)
-      const u32 rowlen = rowk + k - 1;
+      const u32 lim = jend + k - 1;
       u64 p = cbase + j;
-      const u64 end = cbase + rowlen;
-      nN = rowlen;
+      const u64 end = cbase + lim;
+      nN = lim;
       while (p < end) {
-        const u64 x = rd.nmask[p >> 6] & (~0ull >> (p & 63));
-        if (x) {
-          const u64 q = (p & ~63ull) + static_cast<u64>(__clzll(x));
+        const u64 w0 = p >> 6, wl = (end - 1) >> 6;
+        u64 x[4];
+#pragma unroll
+        for (u32 t = 0; t < 4; t++) x[t] = w0 + t <= wl ? rd.nmask[w0 + t] : 0ull;
+        x[0] &= ~0ull >> (p & 63);
+        u32 hit = 4;
+#pragma unroll
+        for (u32 t = 4; t-- > 0;)
+          if (x[t]) hit = t;
+        if (hit < 4) {
+          const u64 q = ((w0 + hit) << 6) + static_cast<u64>(__clzll(x[hit]));
           if (q < end) nN = static_cast<u32>(q - cbase);
           break;
         }
-        p = (p | 63) + 1;
+        p = (w0 + 4) << 6;
       }
     }
     return nN <= j + k - 1;
