@@ -21,9 +21,8 @@
 // Written by the search kernels lane by lane they took the uniform-step
 // kernel from 2.35 to 4.58 ms per batch of 5%-N reads.
 //
-// One thread per row finds the runs from the row's non-ACGT mask, a word at
-// a time: each mask word is read once (the next one while the current one's
-// set bits are consumed in registers). A wave's 64 rows take their item
+// One thread per row finds the runs from the row's non-ACGT mask, four words
+// at a time (loaded together, then their set bits consumed in registers). A wave's 64 rows take their item
 // slots with one atomic (about 10^4 per batch of 650K rows), so items of 64
 // consecutive rows are contiguous and in row order. Earlier versions: slots
 // taken per 64-window chunk (2 million atomics on one counter per batch,
@@ -71,16 +70,21 @@ __device__ __forceinline__ u64 row_items(const u64* __restrict__ nmask, u64 cb, 
     c = q + 1;
   };
   const u64 w0 = cb >> 6, w1 = (end - 1) >> 6;
-  u64 next = nmask[w0];
-  for (u64 w = w0; w <= w1; w++) {
-    u64 x = next;
-    if (w < w1) next = nmask[w + 1];
-    if (w == w0) x &= ~0ull >> (cb & 63);                    // characters before the row
-    if (w == w1 && (end & 63)) x &= ~(~0ull >> (end & 63));  // characters after it
-    while (x) {
-      const u32 b = static_cast<u32>(__clzll(x));
-      stretch_end((w << 6) + b);
-      x &= ~(1ull << (63 - b));
+  for (u64 wb = w0; wb <= w1; wb += 4) {  // four mask words loaded before any is used
+    u64 xs[4];
+#pragma unroll
+    for (u32 t = 0; t < 4; t++) xs[t] = wb + t <= w1 ? nmask[wb + t] : 0ull;
+#pragma unroll
+    for (u32 t = 0; t < 4; t++) {
+      const u64 w = wb + t;
+      u64 x = xs[t];
+      if (w == w0) x &= ~0ull >> (cb & 63);                    // characters before the row
+      if (w == w1 && (end & 63)) x &= ~(~0ull >> (end & 63));  // characters after it
+      while (x) {
+        const u32 b = static_cast<u32>(__clzll(x));
+        stretch_end((w << 6) + b);
+        x &= ~(1ull << (63 - b));
+      }
     }
   }
   stretch_end(end);
