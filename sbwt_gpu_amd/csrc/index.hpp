@@ -88,13 +88,22 @@ struct RunList {
   const unsigned long long* count = nullptr;
   u64 cap = 0;
 };
-// The persistent read search (stream.hip).
+// The persistent read search (stream.hip). item_ctr: the work-item counter
+// of the launch (8 device bytes; null: the index's own). Two searches on one
+// index may overlap only if they count in different places.
 int search_reads_persistent(const sbwt_index* ix, const u64* packed, const u64* nmask, const u64* rs,
                             const u64* off, const u64* work, const u32* items, u64 rows, u64 bound, u32 seg,
                             const u8* maybe, u64* out, bool nbreak, bool stream, bool cert, bool count,
-                            const RunList& runs, hipStream_t s);
+                            const RunList& runs, unsigned long long* item_ctr, hipStream_t s);
 // The read search with run items, as sbwt_search_reads with flags (search.hip).
 int search_reads_runs(const sbwt_index* ix, const u64* packed, const u64* row_starts, const u64* offsets, u64 rows,
-                      u32 seg, const u8* maybe, u64* out, u32 flags, const RunList& runs, hipStream_t s);
+                      u32 seg, const u8* maybe, u64* out, u32 flags, const RunList& runs,
+                      unsigned long long* item_ctr, hipStream_t s);
+// sbwt_search_reads_masked (d_maybe may be null) with the work-item counter
+// of a plan (search.hip).
+int search_reads_counted_on(const sbwt_index* ix, const u64* packed, const u64* nmask, const u64* row_starts,
+                            const u64* offsets, const u64* work, const u32* items, u64 rows, u64 work_bound,
+                            u32 seg, const u8* maybe, u64* out, u32 flags, unsigned long long* item_ctr,
+                            hipStream_t s);
 
 }  // namespace sbwt

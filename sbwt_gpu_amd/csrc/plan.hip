@@ -27,6 +27,9 @@ struct sbwt_reads_plan {
   uint4* d_runs = nullptr;
   unsigned long long* d_run_count = nullptr;
   sbwt::u64 run_cap = 0;
+  // the uniform-step kernel's work-item counter: one per plan, so searches
+  // on different plans may overlap (on different streams) on one index
+  unsigned long long* d_item_ctr = nullptr;
 };
 
 namespace sbwt {
@@ -105,7 +108,8 @@ int sbwt_reads_plan_create(uint64_t max_rows, uint64_t max_chars, uint32_t seg, 
       hipMalloc(&p->d_work, (max_rows + 1) * 8) != hipSuccess ||
       hipMalloc(&p->d_items, bound * 4) != hipSuccess || hipMalloc(&p->d_scan_tmp, p->scan_bytes) != hipSuccess ||
       hipMalloc(&p->d_runs, (p->run_cap = run_items_bound(max_rows, max_chars, seg)) * sizeof(uint4)) != hipSuccess ||
-      hipMalloc(&p->d_run_count, sizeof(unsigned long long)) != hipSuccess) {
+      hipMalloc(&p->d_run_count, sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&p->d_item_ctr, sizeof(unsigned long long)) != hipSuccess) {
     sbwt_reads_plan_free(p);
     return fail(SBWT_ERR_HIP, "out of device memory for a plan of %llu rows, %llu characters",
                 (unsigned long long)max_rows, (unsigned long long)max_chars);
@@ -118,7 +122,8 @@ int sbwt_reads_plan_free(sbwt_reads_plan* p) {
   if (!p) return SBWT_OK;
   for (void* q : {static_cast<void*>(p->d_packed), static_cast<void*>(p->d_nmask), static_cast<void*>(p->d_offsets),
                   static_cast<void*>(p->d_work), static_cast<void*>(p->d_items), p->d_scan_tmp,
-                  static_cast<void*>(p->d_runs), static_cast<void*>(p->d_run_count)})
+                  static_cast<void*>(p->d_runs), static_cast<void*>(p->d_run_count),
+                  static_cast<void*>(p->d_item_ctr)})
     if (q) (void)hipFree(q);
   delete p;
   return SBWT_OK;
@@ -149,14 +154,14 @@ static int plan_search_impl(sbwt_reads_plan* p, const sbwt_index* ix, const uint
     SBWT_TRY(run_items(p->d_nmask, d_row_starts, off, rows, ix->k, p->seg, p->d_runs, p->run_cap, p->d_run_count,
                        d_out, s));
     return search_reads_runs(ix, p->d_packed, d_row_starts, off, rows, p->seg, nullptr, d_out, flags,
-                             RunList{p->d_runs, p->d_run_count, p->run_cap}, s);
+                             RunList{p->d_runs, p->d_run_count, p->run_cap}, p->d_item_ctr, s);
   }
   SBWT_TRY(plan_scan(p, PlanWorkCount{off, rows, p->seg}, rows, p->d_work, s));
   k_plan_items<<<(rows + 255) / 256, 256, 0, s>>>(p->d_work, rows, p->d_items);
   SBWT_HIP(hipGetLastError());
   const u64 bound = sbwt_reads_work_bound(rows, n_chars, p->seg);
-  return sbwt_search_reads(ix, p->d_packed, p->d_nmask, d_row_starts, off, p->d_work, p->d_items, rows, bound,
-                           p->seg, d_out, flags, stream);
+  return search_reads_counted_on(ix, p->d_packed, p->d_nmask, d_row_starts, off, p->d_work, p->d_items, rows,
+                                 bound, p->seg, nullptr, d_out, flags, p->d_item_ctr, s);
 }
 
 }  // namespace sbwt
@@ -194,7 +199,7 @@ constexpr u64 kItemBytes = align256((kStaticRows + kStaticChars / kStaticMinSeg 
 constexpr u64 kRunCap = 2 * kStaticRows + kStaticChars / (kRunItemsMinK + 1) + kStaticChars / kStaticMinSeg + 64;
 constexpr u64 kRunBytes = align256(kRunCap * 16);
 constexpr u64 kScratchBytes =
-    kPackedBytes + kMaskBytes + 2 * kRowBytes + kItemBytes + kScanReserve + kRunBytes + 256;
+    kPackedBytes + kMaskBytes + 2 * kRowBytes + kItemBytes + kScanReserve + kRunBytes + 2 * 256;
 
 __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
@@ -262,5 +267,7 @@ extern "C" int sbwt_reads_search_static(const sbwt_index* ix, const uint8_t* d_c
   v.run_cap = kRunCap;
   b += kRunBytes;
   v.d_run_count = reinterpret_cast<unsigned long long*>(b);
+  b += 256;
+  v.d_item_ctr = reinterpret_cast<unsigned long long*>(b);
   return plan_search_impl(&v, ix, d_chars, n_chars, d_row_starts, rows, d_out, d_offsets_out, flags, stream);
 }

@@ -523,7 +523,8 @@ static int launch_reads(const sbwt_index* ix, const ReadArgs& a, bool stream, bo
 }
 
 int search_reads_runs(const sbwt_index* ix, const u64* packed, const u64* row_starts, const u64* offsets, u64 rows,
-                      u32 seg, const u8* maybe, u64* out, u32 flags, const RunList& runs, hipStream_t s) {
+                      u32 seg, const u8* maybe, u64* out, u32 flags, const RunList& runs,
+                      unsigned long long* item_ctr, hipStream_t s) {
   if (rows == 0 || runs.cap == 0) return SBWT_OK;
   // run items hold no non-ACGT window: nothing to break on
   const bool independent = (flags & SBWT_READS_INDEPENDENT) != 0;
@@ -534,7 +535,7 @@ int search_reads_runs(const sbwt_index* ix, const u64* packed, const u64* row_st
                        (!(flags & SBWT_READS_SEGMENT_LANES) && stream_mode && cert);
   if (uniform)
     return search_reads_persistent(ix, packed, nullptr, row_starts, offsets, nullptr, nullptr, rows, runs.cap, seg,
-                                   maybe, out, false, stream_mode, cert, cnt, runs, s);
+                                   maybe, out, false, stream_mode, cert, cnt, runs, item_ctr, s);
   const ReadArgs a{packed, nullptr, row_starts, offsets, nullptr, rows, runs.cap, seg, maybe, out, runs};
   switch (ix->layout) {
     case 16: return launch_reads<16>(ix, a, stream_mode, false, cert, cnt, s);
@@ -601,7 +602,8 @@ int sbwt_search_positions(const sbwt_index* ix, const uint64_t* d_packed, uint64
 static int search_reads_impl(const sbwt_index* ix, const uint64_t* d_packed, const uint64_t* d_nmask,
                              const uint64_t* d_row_starts, const uint64_t* d_offsets, const uint64_t* d_work,
                              const uint32_t* d_items, uint64_t rows, uint64_t work_bound, uint32_t seg,
-                             const uint8_t* d_maybe, uint64_t* d_out, uint32_t flags, void* stream) {
+                             const uint8_t* d_maybe, uint64_t* d_out, uint32_t flags, void* stream,
+                             unsigned long long* item_ctr = nullptr) {
   SBWT_REQUIRE(ix, "index is NULL");
   if (rows == 0 || work_bound == 0) return SBWT_OK;
   SBWT_REQUIRE(d_packed && d_row_starts && d_offsets && d_work && d_out, "device buffers must not be NULL");
@@ -634,7 +636,7 @@ static int search_reads_impl(const sbwt_index* ix, const uint64_t* d_packed, con
   if (uniform)
     return search_reads_persistent(ix, d_packed, d_nmask, d_row_starts, d_offsets, d_work, d_items, rows,
                                    work_bound, seg, d_maybe, d_out, nbreak, stream_mode, cert, cnt, RunList{},
-                                   as_stream(stream));
+                                   item_ctr, as_stream(stream));
   const ReadArgs a{d_packed, d_nmask, d_row_starts, d_offsets, d_work, rows, work_bound, seg, d_maybe, d_out,
                    RunList{}};
   hipStream_t s = as_stream(stream);
@@ -665,3 +667,15 @@ int sbwt_search_reads_masked(const sbwt_index* ix, const uint64_t* d_packed, con
 }
 
 }  // extern "C"
+
+namespace sbwt {
+
+int search_reads_counted_on(const sbwt_index* ix, const u64* packed, const u64* nmask, const u64* row_starts,
+                            const u64* offsets, const u64* work, const u32* items, u64 rows, u64 work_bound,
+                            u32 seg, const u8* maybe, u64* out, u32 flags, unsigned long long* item_ctr,
+                            hipStream_t s) {
+  return search_reads_impl(ix, packed, nmask, row_starts, offsets, work, items, rows, work_bound, seg, maybe, out,
+                           flags, s, item_ctr);
+}
+
+}  // namespace sbwt

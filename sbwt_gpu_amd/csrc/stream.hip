@@ -628,7 +628,8 @@ __global__ __launch_bounds__(256) void k_reads_uniform(DevIndex ix, ReadsDev rd,
 constexpr int kWalkersPerLane = 1;
 
 template <int CB, bool COUNT>
-static int launch_uniform(const sbwt_index* ix, const ReadsDev& rd, u64 bound, u64* out, hipStream_t s) {
+static int launch_uniform(const sbwt_index* ix, const ReadsDev& rd, u64 bound, u64* out,
+                          unsigned long long* item_ctr, hipStream_t s) {
   const u32 bt = kAnswerStride;  // the answer buffer's LDS layout assumes this block size
   constexpr int P = kWalkersPerLane;
   int per = 0;
@@ -639,8 +640,9 @@ static int launch_uniform(const sbwt_index* ix, const ReadsDev& rd, u64 bound, u
   const u64 cap = static_cast<u64>(ix->num_cus ? ix->num_cus : 256) * static_cast<u64>(per);
   if (grid > cap) grid = cap;
   if (grid == 0) grid = 1;
-  k_zero_counter<<<1, 1, 0, s>>>(ix->d_item_ctr);
-  k_reads_uniform<CB, COUNT, P><<<grid, bt, 0, s>>>(dev_view(ix), rd, ix->d_item_ctr, out, ix->d_counters);
+  unsigned long long* ctr = item_ctr ? item_ctr : ix->d_item_ctr;
+  k_zero_counter<<<1, 1, 0, s>>>(ctr);
+  k_reads_uniform<CB, COUNT, P><<<grid, bt, 0, s>>>(dev_view(ix), rd, ctr, out, ix->d_counters);
   SBWT_HIP(hipGetLastError());
   return SBWT_OK;
 }
@@ -648,14 +650,17 @@ static int launch_uniform(const sbwt_index* ix, const ReadsDev& rd, u64 bound, u
 int search_reads_persistent(const sbwt_index* ix, const u64* packed, const u64* nmask, const u64* rs,
                             const u64* off, const u64* work, const u32* items, u64 rows, u64 bound, u32 seg,
                             const u8* maybe, u64* out, bool nbreak, bool stream, bool cert, bool count,
-                            const RunList& runs, hipStream_t s) {
+                            const RunList& runs, unsigned long long* item_ctr, hipStream_t s) {
   const ReadsDev rd{packed, nmask, rs, off, work, items, maybe, rows, seg,
                     nbreak ? 1u : 0u, stream ? 1u : 0u, cert ? 1u : 0u,
                     (reinterpret_cast<uintptr_t>(out) & 15) == 0 ? 1u : 0u, runs.runs, runs.count, runs.cap};
   switch (ix->layout) {
-    case 16: return count ? launch_uniform<16, true>(ix, rd, bound, out, s) : launch_uniform<16, false>(ix, rd, bound, out, s);
-    case 32: return count ? launch_uniform<32, true>(ix, rd, bound, out, s) : launch_uniform<32, false>(ix, rd, bound, out, s);
-    default: return count ? launch_uniform<64, true>(ix, rd, bound, out, s) : launch_uniform<64, false>(ix, rd, bound, out, s);
+    case 16: return count ? launch_uniform<16, true>(ix, rd, bound, out, item_ctr, s)
+                          : launch_uniform<16, false>(ix, rd, bound, out, item_ctr, s);
+    case 32: return count ? launch_uniform<32, true>(ix, rd, bound, out, item_ctr, s)
+                          : launch_uniform<32, false>(ix, rd, bound, out, item_ctr, s);
+    default: return count ? launch_uniform<64, true>(ix, rd, bound, out, item_ctr, s)
+                          : launch_uniform<64, false>(ix, rd, bound, out, item_ctr, s);
   }
 }
 

@@ -77,6 +77,48 @@ def test_plan_batches_match_oracle_and_graph_replays(persistent):
         assert torch.equal(out, eager), f"replay {rep}"
 
 
+@pytest.mark.parametrize("kw", [{"persistent": True}, {"persistent": True, "runs": True}, {"lanes": True}])
+def test_two_plans_on_two_streams_overlap_safely(kw):
+    """Batches alternate over two plans on two streams, so the searches of
+    one index overlap (each plan counts its own work items): eagerly and in
+    one captured graph, every replay must equal the oracle."""
+    rng, s, ix, oix, chars, starts, set_starts = _setup(seed=8)
+    k = 31
+    sh = starts.cpu().numpy()
+    kc = np.maximum(np.diff(sh) - k + 1, 0)
+    kcum = np.concatenate([[0], np.cumsum(kc)])
+    want = O.search_reads(oix, chars.cpu().numpy(), sh.astype(np.uint64), k)
+    batches = _batches(chars, starts, set_starts, per=1)
+    mr, mc = max(b[1].numel() - 1 for b in batches), max(b[0].numel() for b in batches)
+    plans = [sg.ReadsPlan(mr, mc, seg=64) for _ in range(2)]
+    lanes = [torch.cuda.Stream() for _ in range(2)]
+    out = torch.full((int(kcum[-1]),), 7, dtype=torch.int64, device="cuda")
+
+    def run_all(base):
+        for st in lanes:
+            st.wait_stream(base)
+        for i, (bc, brel, b0, b1) in enumerate(batches):
+            plans[i % 2].search(ix, bc, brel, out[int(kcum[b0]):int(kcum[b1])], stream=lanes[i % 2], **kw)
+        for st in lanes:
+            base.wait_stream(st)
+
+    run_all(torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(sg.to_u64(out), want)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=side):
+        run_all(side)
+    torch.cuda.synchronize()
+    for rep in range(3):
+        out.fill_(-7)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(sg.to_u64(out), want, err_msg=f"replay {rep}")
+
+
 def test_plan_rejects_oversized_batch():
     rng, s, ix, oix, chars, starts, set_starts = _setup(seed=2)
     plan = sg.ReadsPlan(3, 100)
