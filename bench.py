@@ -293,6 +293,41 @@ def run_reads(args) -> dict:
         roofline["working_set_random_roof"] = ws
         roofline["frac_of_working_set_roof"] = round(achieved / ws["GBps_128B_lines"], 4)
 
+    # ---- consecutive batches over several plans and streams -----------------
+    # Step i on plan i % S and stream i % S (each plan counts its own work
+    # items): one batch's tail overlaps the next batch's search. Each stream
+    # writes its own answers, checked against the single-stream ones.
+    pipelined = None
+    S = max(1, args.streams)
+    if S > 1 and args.mode == "default":
+        plans = [sg.ReadsPlan(rows, n_chars, args.seg) for _ in range(S)]
+        lanes = [torch.cuda.Stream() for _ in range(S)]
+        outs = [torch.empty_like(out) for _ in range(S)]
+
+        def batches(n):
+            for st in lanes:
+                st.wait_stream(stream)
+            for i in range(n):
+                plans[i % S].search(ix, rchars, rstarts, outs[i % S], stream=lanes[i % S])
+            for st in lanes:
+                stream.wait_stream(st)
+
+        batches(max(args.warmup, S))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        batches(args.steps)
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / args.steps
+        same_p = all(np.array_equal(o.cpu().numpy().view(np.uint64), res_gpu) for o in outs)
+        pipelined = {"streams": S, "kmers_per_s": round(total / el, 1), "ms_per_step": round(el * 1e3, 4),
+                     "answers_equal_single_stream": same_p,
+                     "lines_per_s_GBps": round(lines * 128 / el / 1e9, 1)}
+        log(f"{S} plans and streams: {total / el / 1e9:.3f} G k-mers/s ({el * 1e3:.3f} ms per batch); "
+            f"answers equal: {same_p}")
+        for p_ in plans:
+            p_.close()
+        del outs
+
     # ---- CPU baseline + parity on a sample -----------------------------------
     cpu = None
     if not args.no_cpu:
@@ -309,6 +344,7 @@ def run_reads(args) -> dict:
                    "kmers_per_step": total, "index_columns": info["n"], "layout": f"L{info['layout']}",
                    "mode": args.mode, "segment_kmers": args.seg,
                    "search_kernel_kmers_per_s": round(total / kern_avg_s, 1),
+                   "consecutive_batches_on_streams": pipelined,
                    "found_fraction": round(found / total, 4),
                    "h2d_reads_GBps": round(rchars_h.nbytes / h2d_s / 1e9, 2)},
         "roofline": roofline,
