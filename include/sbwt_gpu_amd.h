@@ -444,7 +444,9 @@ int sbwt_reads_plan_free(sbwt_reads_plan *plan);
  * sbwt_search_reads with `flags`. d_out needs one entry per k-mer (at most
  * n_chars). Allocates nothing and does not synchronize, so a sequence of
  * batches can be captured in one HIP graph and replayed; calls sharing a
- * plan must be ordered on one stream. */
+ * plan must be ordered on one stream. Each plan counts its own work items,
+ * so calls on different plans may overlap on different streams, also on one
+ * index (sbwt_search_reads calls on one index may not). */
 int sbwt_reads_plan_search(sbwt_reads_plan *plan, const sbwt_index *ix,
                            const uint8_t *d_chars, uint64_t n_chars,
                            const uint64_t *d_row_starts, uint64_t rows,
