@@ -34,12 +34,12 @@ template <u32 S>
 struct AnswerBufferT {
   static_assert(S >= 2 && S <= 16 && (S & (S - 1)) == 0, "a power of two, whole 16-byte stores");
   u64* obuf;  // this thread's slot 0 in LDS
-  u64 group = ~0ull;
   u32 mask = 0;
 
-  __device__ __forceinline__ void flush(u64* __restrict__ out, bool vec) {
+  // Writes the buffered answers of the group holding out[q].
+  __device__ __forceinline__ void flush(u64* __restrict__ out, bool vec, u64 q) {
     if (!mask) return;
-    u64* o = out + group * S;
+    u64* o = out + (q & ~static_cast<u64>(S - 1));
     if (mask == (1u << S) - 1 && vec) {
 #pragma unroll
       for (u32 h = 0; h < S / 2; h++) {
@@ -63,8 +63,7 @@ struct AnswerBufferT {
     const u32 sl = static_cast<u32>(q) & (S - 1);
     obuf[sl * kAnswerStride] = v;
     mask |= 1u << sl;
-    group = q / S;
-    if (sl == S - 1 || last) flush(out, vec);
+    if (sl == S - 1 || last) flush(out, vec, q);
   }
 
   // out[q .. q+n-1] = v (n >= 1); `last`: the lane's item ends at q+n-1.
@@ -79,10 +78,9 @@ struct AnswerBufferT {
       for (u32 t = 0; t < S; t++)
         if (t >= sl && t < sl + c) obuf[t * kAnswerStride] = v;
       mask |= ((1u << c) - 1u) << sl;
-      group = q / S;
+      if (sl + c == S || (last && n == c)) flush(out, vec, q);
       q += c;
       n -= c;
-      if (sl + c == S || (last && n == 0)) flush(out, vec);
     }
     const uint4 vv = make_uint4(static_cast<u32>(v), static_cast<u32>(v >> 32), static_cast<u32>(v),
                                 static_cast<u32>(v >> 32));
@@ -101,8 +99,7 @@ struct AnswerBufferT {
       for (u32 t = 0; t < S; t++)
         if (t < n) obuf[t * kAnswerStride] = v;
       mask |= (1u << n) - 1u;
-      group = q / S;
-      if (last) flush(out, vec);
+      if (last) flush(out, vec, q);
     }
   }
 };

@@ -48,17 +48,22 @@ __device__ __forceinline__ void load_chunk(const uint4* __restrict__ row, u32 ch
   }
 }
 
-// Number of 1-bits before offset o (0 <= o < COLS) plus the chunk's count.
+// Number of 1-bits before offset o (0 <= o < COLS) plus the chunk's count:
+// the whole payload words below o's word, then o's word under a mask of its
+// o % 32 low bits. The word comparisons are shared by the whole-word counts
+// and the selection of o's word, so a rank in a 16-byte chunk is about 11
+// vector instructions (a mask per word took about 28).
 template <int CB>
 __device__ __forceinline__ u32 rank_chunk(const Chunk<CB>& c, u32 o) {
   u32 r = c.w[0];
+  u32 part = c.w[1];
 #pragma unroll
-  for (u32 t = 0; t < Layout<CB>::PW; t++) {
-    const u32 lo = 32 * t;
-    const u32 m = (o >= lo + 32) ? ~0u : ((o > lo) ? ((1u << (o - lo)) - 1u) : 0u);
-    r += __popc(c.w[1 + t] & m);
+  for (u32 t = 1; t < Layout<CB>::PW; t++) {
+    const bool past = o >= 32 * t;  // word t - 1 lies wholly below o
+    r += __popc(past ? c.w[t] : 0u);
+    part = past ? c.w[1 + t] : part;
   }
-  return r;
+  return r + __popc(part & ((1u << (o & 31)) - 1u));
 }
 
 template <int CB>

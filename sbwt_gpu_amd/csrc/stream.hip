@@ -43,6 +43,8 @@ enum : u32 { M_IDLE = 0, M_SEARCH, M_STREAM, M_EDGE, M_GSCAN };
 // P_CERTS: a P_CERTP search from the dead end of a k-mer search that died
 // early (see start_cert)
 enum : u32 { P_KMER = 0, P_CERTP, P_CERTL, P_CERTS };
+// a purpose flag: the search's window is still to be loaded
+constexpr u32 kWinPending = 4;
 constexpr u32 kNoPos = 0xFFFFFFFFu;
 // Work items a wave claims per atomic on the shared counter (at least 64,
 // so one claim serves every lane of a refill). One atomic per lane or per
@@ -65,6 +67,29 @@ struct ReadsDev {
   const uint4* runs;  // run items {row, first k-mer, k-mers}, or null: segments of the work prefix
   const unsigned long long* run_count;
   u64 run_cap;  // items the run buffer holds
+};
+
+// The kernel's variants. FX_ANY reads every option from ReadsDev at run
+// time. The two read-search defaults fix them at compile time, which drops
+// their branches and frees the scalar registers that would hold them:
+// FX_SEGMENTS streams with certificates over segment items given by an item
+// array, breaks k-mers at non-ACGT characters, has no prefilter flags and
+// writes 16-byte-aligned answers; FX_RUNS does the same over run items
+// (which hold no non-ACGT window, so nothing breaks).
+enum : int { FX_ANY = 0, FX_SEGMENTS = 1, FX_RUNS = 2 };
+
+template <int FX>
+struct Fx {
+  __device__ __forceinline__ static bool nbreak(const ReadsDev& rd) {
+    return FX == FX_SEGMENTS ? true : (FX == FX_RUNS ? false : rd.nbreak != 0);
+  }
+  __device__ __forceinline__ static bool stream(const ReadsDev& rd) { return FX != FX_ANY || rd.stream != 0; }
+  __device__ __forceinline__ static bool cert(const ReadsDev& rd) { return FX != FX_ANY || rd.cert != 0; }
+  __device__ __forceinline__ static bool vec(const ReadsDev& rd) { return FX != FX_ANY || rd.vec_out != 0; }
+  __device__ __forceinline__ static const u8* maybe(const ReadsDev& rd) { return FX == FX_ANY ? rd.maybe : nullptr; }
+  __device__ __forceinline__ static bool runs(const ReadsDev& rd) {
+    return FX == FX_RUNS ? true : (FX == FX_SEGMENTS ? false : rd.runs != nullptr);
+  }
 };
 
 __device__ __forceinline__ u64 window2(const u64* __restrict__ w, u64 pos) {
@@ -95,8 +120,9 @@ constexpr u32 answer_slots() {
   return CB == 16 ? 8u : kAnswerSlots;
 }
 
-template <int CB, bool COUNT>
+template <int CB, bool COUNT, int FX>
 struct Walker {
+  using F = Fx<FX>;
   static constexpr u32 COLS = Layout<CB>::COLS;
   using Ans = AnswerBufferT<answer_slots<CB>()>;
   // the item: k-mers [j, jend) of a row of rowk k-mers starting at character cbase
@@ -107,13 +133,23 @@ struct Walker {
   // the current step
   u32 mode = M_IDLE, purpose = P_KMER;
   u32 l = 0, r = 0, i = 0, len = 0;  // search state (M_GSCAN: l = chunk, r = last offset)
-  u64 win = 0;                             // top-aligned window of the searched string
+  // top-aligned window of the searched string (while purpose has kWinPending:
+  // the string's row-relative start, the window not loaded yet)
+  u64 win = 0;
   bool prev = false;
   u32 pnode = 0, sc = 0;
   // certified-absent k-mers [clo, chi]; a k-mer at kpos already found at knode
   u32 clo = 1, chi = 0, kpos = kNoPos, knode = 0, cp = 0;
   u32 nN = kNoPos;  // first non-ACGT position (row-relative) at or after j, if >= j; kNoPos = unknown
-  u64 cw = 0, cw_idx = ~0ull;  // packed word holding the last streamed character
+  // The packed words wi and wi + 1 of the batch (characters 32 wi .. 32 wi
+  // + 63), wi = the word of the k-mer at j: they hold both the next streamed
+  // character (j + k - 1 <= 32 wi + 62) and the window of a search from j.
+  // wn is word wi + 2, loaded when wi moves on by one word and not read
+  // before wi moves again (32 k-mers later when streaming), so a wave never
+  // waits for it: the load overlaps the steps instead of delaying one.
+  // (wi = ~0 - 1: no words yet, as after a change of item; wi + 1 is never a
+  // valid word.)
+  u64 w0 = 0, w1 = 0, wn = 0, wi = ~0ull - 1;
   MaybeFlags mf;               // prefilter flags of the item's k-mers
   // One step's reads: chunks at columns a and b of one row, or a presearch
   // entry. Only the addresses live across the loads; update() recomputes the
@@ -143,10 +179,10 @@ struct Walker {
   __device__ __forceinline__ void setup(const ReadsDev& rd, u64 it) {
     u64 row = 0;
     uint4 run = make_uint4(0, 0, 0, 0);
-    if (rd.runs) {
+    if (F::runs(rd)) {
       run = rd.runs[it];
       row = run.x;
-    } else if (rd.items) {
+    } else if (FX == FX_SEGMENTS || rd.items) {
       row = rd.items[it];
     } else {
       u64 hi = rd.rows;
@@ -157,7 +193,7 @@ struct Walker {
     }
     const u64 o0 = rd.offsets[row];
     rowk = static_cast<u32>(rd.offsets[row + 1] - o0);
-    if (rd.runs) {
+    if (F::runs(rd)) {
       j = run.y;
       jv = run.z;
       jend = run.w;
@@ -168,12 +204,13 @@ struct Walker {
     }
     cbase = rd.row_starts[row];
     obase = o0;
+    wi = ~0ull - 1;  // the words held belong to the last row (wn only if that row reached it)
     prev = false;
     clo = 1;
     chi = 0;
     kpos = kNoPos;
     nN = kNoPos;
-    if (rd.maybe) mf.reset(rd.maybe, obase + j);
+    if (F::maybe(rd)) mf.reset(F::maybe(rd), obase + j);
   }
 
   // k-mers [from, to) of the row of rk k-mers at characters cb, answers at
@@ -183,6 +220,7 @@ struct Walker {
   __device__ __forceinline__ void take(const ReadsDev& rd, u64 cb, u64 ob, u32 rk, u32 from, u32 to, u32 gap_end) {
     cbase = cb;
     obase = ob;
+    wi = ~0ull - 1;
     rowk = rk;
     j = from;
     jv = gap_end;
@@ -193,14 +231,15 @@ struct Walker {
     chi = 0;
     kpos = kNoPos;
     nN = kNoPos;
-    if (rd.maybe) mf.reset(rd.maybe, obase + j);
+    if (F::maybe(rd)) mf.reset(F::maybe(rd), obase + j);
   }
 
-  __device__ __forceinline__ void start_search(const DevIndex& ix, const ReadsDev& rd, UTally& t, u32 pur,
-                                               u32 start, u32 length) {
-    purpose = pur;
+  // a search of `length` characters whose top-aligned window is `window`
+  // (with pur | kWinPending: whose row-relative start is `window`)
+  __device__ __forceinline__ void begin_search(const DevIndex& ix, UTally& t, u32 pur, u32 length, u64 window) {
     len = length;
-    win = window2(rd.packed, cbase + start);
+    purpose = pur;
+    win = window;
     i = 0;
     mode = M_SEARCH;
     if (COUNT) t.searches++;
@@ -210,10 +249,16 @@ struct Walker {
     }
   }
 
+  // a search of the string of `length` characters at row position start;
+  // its window is loaded by load_window()
+  __device__ __forceinline__ void start_search(const DevIndex& ix, UTally& t, u32 pur, u32 start, u32 length) {
+    begin_search(ix, t, pur | kWinPending, length, start);
+  }
+
   // the answer for the k-mer at j (through the lane's answer buffer); moves to j + 1
   __device__ __forceinline__ void answer(u64* __restrict__ out, const ReadsDev& rd, Ans& ab, UTally& t,
                                          u64 v) {
-    ab.put(out, rd.vec_out != 0, obase + j, v, j + 1 >= jend);
+    ab.put(out, F::vec(rd), obase + j, v, j + 1 >= jend);
     j++;
     if (COUNT) t.kmers++;
   }
@@ -221,7 +266,7 @@ struct Walker {
   // v for the n k-mers from j on; moves to j + n
   __device__ __forceinline__ void answer_run(u64* __restrict__ out, const ReadsDev& rd, Ans& ab,
                                              UTally& t, u32 n, u64 v) {
-    ab.put_run(out, rd.vec_out != 0, obase + j, n, v, j + n >= jend);
+    ab.put_run(out, F::vec(rd), obase + j, n, v, j + n >= jend);
     j += n;
     if (COUNT) t.kmers += n;
   }
@@ -238,7 +283,7 @@ struct Walker {
   // (shrt: by a search that died within dead_len characters)
   __device__ __forceinline__ void start_cert(const DevIndex& ix, const ReadsDev& rd, UTally& t, u32 d,
                                              bool shrt) {
-    if (!rd.cert) return;
+    if (!F::cert(rd)) return;
     if (clo <= chi && chi >= j) return;  // a pending range still covers what follows
     if (d < j) return;                   // nothing ahead to certify
     if (d >= rowk) {
@@ -256,7 +301,7 @@ struct Walker {
       const u32 s = max(q, d + 1 >= ls ? d + 1 - ls : 0u);
       chi = q;
       clo = q + 1;  // empty until the search proves it (P_CERTL then sets clo = j)
-      start_search(ix, rd, t, P_CERTL, s, d - s + 1);
+      start_search(ix, t, P_CERTL, s, d - s + 1);
       return;
     }
     // The search from d certifies only if it dies, which a string starting
@@ -266,7 +311,19 @@ struct Walker {
     // on to k characters would cost a whole search for one known k-mer.
     cp = d;
     const u32 cap = dead_len(ix) + 2;
-    start_search(ix, rd, t, shrt ? P_CERTS : P_CERTP, d, cap < ix.k ? cap : ix.k);
+    start_search(ix, t, shrt ? P_CERTS : P_CERTP, d, cap < ix.k ? cap : ix.k);
+  }
+
+  // The window of a search started since the last step. Searches start at
+  // four places (a k-mer from scratch, two certificate kinds, the shared
+  // string after a dead certificate); loading their windows here, at one
+  // place per iteration, keeps a wave from running the load code once per
+  // place that some lane reached.
+  __device__ __forceinline__ void load_window(const ReadsDev& rd) {
+    if (purpose & kWinPending) {
+      win = window2(rd.packed, cbase + static_cast<u32>(win));
+      purpose &= ~kWinPending;
+    }
   }
 
   // does the k-mer at j contain a non-ACGT character? nN = the first
@@ -317,23 +374,38 @@ struct Walker {
       // character holds it), or filtered out
       u32 run = 0;
       if (j >= clo && j <= chi) run = chi + 1 - j;
-      else if (rd.nbreak && has_n(rd, k)) run = nN + 1 - j;
-      else if (rd.maybe) run = mf.zeros(rd.maybe, nk, obase + j);
+      else if (F::nbreak(rd) && has_n(rd, k)) run = nN + 1 - j;
+      else if (F::maybe(rd)) run = mf.zeros(F::maybe(rd), nk, obase + j);
       const bool known = j == kpos;  // found by a certificate search
       if (run || known) {
         prev = !run;
         pnode = knode;
         answer_run(out, rd, ab, t, run ? (run < jend - j ? run : jend - j) : 1u, run ? kNone : knode);
-      } else if (rd.stream && prev) {
-        const u64 p = cbase + j + k - 1;  // the new character
-        if ((p >> 5) != cw_idx) {
-          cw_idx = p >> 5;
-          cw = rd.packed[cw_idx];
-        }
-        sc = static_cast<u32>(cw >> (62 - 2 * (p & 31))) & 3u;
-        mode = M_STREAM;
       } else {
-        start_search(ix, rd, t, P_KMER, j, k);
+        const u64 at = cbase + j;
+        const u64 wt = at >> 5;
+        if (wt != wi) {
+          if (wt == wi + 1) {
+            w0 = w1;
+            w1 = wn;
+          } else {
+            w0 = rd.packed[wt];
+            w1 = rd.packed[wt + 1];
+          }
+          // (only when the row reaches that word: the batch's packed array
+          // ends one word after its last character's word)
+          wn = ((wt + 2) << 5) <= cbase + rowk + k - 2 ? rd.packed[wt + 2] : 0ull;
+          wi = wt;
+        }
+        const u32 o = static_cast<u32>(at & 31);  // j's offset in w0
+        if (F::stream(rd) && prev) {
+          const u32 q = o + k - 1;  // the new character, at most 62
+          sc = static_cast<u32>((q < 32 ? w0 >> (62 - 2 * q) : w1 >> (126 - 2 * q))) & 3u;
+          mode = M_STREAM;
+        } else {
+          // the window of the k-mer at j, from the two words
+          begin_search(ix, t, P_KMER, k, o ? (w0 << (2 * o)) | (w1 >> (64 - 2 * o)) : w0);
+        }
       }
     }
   }
@@ -357,8 +429,12 @@ struct Walker {
     // later parts are unused and must not step past the table)
 #pragma unroll
     for (u32 t = 0; t < Layout<CB>::Q; t++) {
+      // the second chunk only where it differs from the first (a streaming
+      // step's j and j + 1 share a chunk but at its end; so does a narrow
+      // interval): no second request for the same line
       const uint4 va = st.pa[st.pre ? 0 : t];
-      const uint4 vb = st.pb[st.pre ? 0 : t];
+      uint4 vb = va;
+      if (st.pb != st.pa) vb = st.pb[t];
       qa.w[4 * t] = va.x;
       qa.w[4 * t + 1] = va.y;
       qa.w[4 * t + 2] = va.z;
@@ -437,7 +513,7 @@ struct Walker {
             const u32 lo = cp + i > k ? cp + i - k : 0u;
             clo = lo > j ? lo : j;
             chi = cp;
-            if (lo > j && cp - lo + 2 <= k) start_search(ix, rd, t, P_CERTL, lo - 1, cp - lo + 2);  // [lo-1, cp]
+            if (lo > j && cp - lo + 2 <= k) start_search(ix, t, P_CERTL, lo - 1, cp - lo + 2);  // [lo-1, cp]
           }
         } else if (sdead) {  // P_CERTL: the k-mers j..lo-1 all contain [lo-1, cp]
           clo = j;
@@ -568,17 +644,20 @@ __device__ __forceinline__ void donate(W& w, const ReadsDev& rd, u32 lane) {
 
 // Each lane walks P work items at once; every iteration issues all P steps'
 // reads before using any, so P dependent chains are in flight per lane.
-template <int CB, bool COUNT, int P>
-__global__ __launch_bounds__(256) void k_reads_uniform(DevIndex ix, ReadsDev rd,
+// Six waves per SIMD for 16- and 32-byte chunks (80 VGPRs; the scalar
+// registers allow no more): unbounded, the compiler took 82-84 and the
+// kernel ran at five.
+template <int CB, bool COUNT, int P, int FX>
+__global__ __launch_bounds__(256, CB == 64 ? 1 : 6) void k_reads_uniform(DevIndex ix, ReadsDev rd,
                                                        unsigned long long* __restrict__ item_ctr,
                                                        u64* __restrict__ out, unsigned long long* ctr) {
-  using W = Walker<CB, COUNT>;
+  using W = Walker<CB, COUNT, FX>;
   static_assert(P == 1, "one LDS answer buffer per lane");
   __shared__ u64 obuf[answer_slots<CB>() * kAnswerStride];
   typename W::Ans ab{obuf + threadIdx.x};
   const u32 lane = threadIdx.x & 63;
-  const u64 total_items = rd.runs ? min(static_cast<u64>(*rd.run_count), rd.run_cap) : (rd.rows ? rd.work[rd.rows] : 0);
-  const u64 nk = rd.maybe && rd.rows ? rd.offsets[rd.rows] : 0;
+  const u64 total_items = Fx<FX>::runs(rd) ? min(static_cast<u64>(*rd.run_count), rd.run_cap) : (rd.rows ? rd.work[rd.rows] : 0);
+  const u64 nk = Fx<FX>::maybe(rd) && rd.rows ? rd.offsets[rd.rows] : 0;
   W w[P];
   UTally t;
   ItemPool pool;
@@ -596,6 +675,7 @@ __global__ __launch_bounds__(256) void k_reads_uniform(DevIndex ix, ReadsDev rd,
       refill(w[p], pool, item_ctr, total_items, rd, lane);
       donate(w[p], rd, lane);
       w[p].advance(ix, rd, nk, out, ab, t);
+      w[p].load_window(rd);
       active = active || w[p].mode != M_IDLE || w[p].live;
     }
     if (!__any(active)) break;
@@ -627,13 +707,13 @@ __global__ __launch_bounds__(256) void k_reads_uniform(DevIndex ix, ReadsDev rd,
 // lane) measured slower on the 100 Mbp read batch: 6.6 ms against 5.8 ms.
 constexpr int kWalkersPerLane = 1;
 
-template <int CB, bool COUNT>
+template <int CB, bool COUNT, int FX>
 static int launch_uniform(const sbwt_index* ix, const ReadsDev& rd, u64 bound, u64* out,
                           unsigned long long* item_ctr, hipStream_t s) {
   const u32 bt = kAnswerStride;  // the answer buffer's LDS layout assumes this block size
   constexpr int P = kWalkersPerLane;
   int per = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_reads_uniform<CB, COUNT, P>, bt, 0) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_reads_uniform<CB, COUNT, P, FX>, bt, 0) != hipSuccess ||
       per < 1)
     per = 4;
   u64 grid = (bound + bt - 1) / bt;
@@ -642,9 +722,22 @@ static int launch_uniform(const sbwt_index* ix, const ReadsDev& rd, u64 bound, u
   if (grid == 0) grid = 1;
   unsigned long long* ctr = item_ctr ? item_ctr : ix->d_item_ctr;
   k_zero_counter<<<1, 1, 0, s>>>(ctr);
-  k_reads_uniform<CB, COUNT, P><<<grid, bt, 0, s>>>(dev_view(ix), rd, ctr, out, ix->d_counters);
+  k_reads_uniform<CB, COUNT, P, FX><<<grid, bt, 0, s>>>(dev_view(ix), rd, ctr, out, ix->d_counters);
   SBWT_HIP(hipGetLastError());
   return SBWT_OK;
+}
+
+template <int CB>
+static int launch_uniform_fx(const sbwt_index* ix, const ReadsDev& rd, u64 bound, u64* out,
+                             unsigned long long* item_ctr, bool count, int fx, hipStream_t s) {
+  switch (fx) {
+    case FX_SEGMENTS: return count ? launch_uniform<CB, true, FX_SEGMENTS>(ix, rd, bound, out, item_ctr, s)
+                                   : launch_uniform<CB, false, FX_SEGMENTS>(ix, rd, bound, out, item_ctr, s);
+    case FX_RUNS: return count ? launch_uniform<CB, true, FX_RUNS>(ix, rd, bound, out, item_ctr, s)
+                               : launch_uniform<CB, false, FX_RUNS>(ix, rd, bound, out, item_ctr, s);
+    default: return count ? launch_uniform<CB, true, FX_ANY>(ix, rd, bound, out, item_ctr, s)
+                          : launch_uniform<CB, false, FX_ANY>(ix, rd, bound, out, item_ctr, s);
+  }
 }
 
 int search_reads_persistent(const sbwt_index* ix, const u64* packed, const u64* nmask, const u64* rs,
@@ -654,13 +747,12 @@ int search_reads_persistent(const sbwt_index* ix, const u64* packed, const u64* 
   const ReadsDev rd{packed, nmask, rs, off, work, items, maybe, rows, seg,
                     nbreak ? 1u : 0u, stream ? 1u : 0u, cert ? 1u : 0u,
                     (reinterpret_cast<uintptr_t>(out) & 15) == 0 ? 1u : 0u, runs.runs, runs.count, runs.cap};
+  const bool fixed = stream && cert && !maybe && rd.vec_out;
+  const int fx = !fixed ? FX_ANY : (runs.runs ? FX_RUNS : (nbreak && items ? FX_SEGMENTS : FX_ANY));
   switch (ix->layout) {
-    case 16: return count ? launch_uniform<16, true>(ix, rd, bound, out, item_ctr, s)
-                          : launch_uniform<16, false>(ix, rd, bound, out, item_ctr, s);
-    case 32: return count ? launch_uniform<32, true>(ix, rd, bound, out, item_ctr, s)
-                          : launch_uniform<32, false>(ix, rd, bound, out, item_ctr, s);
-    default: return count ? launch_uniform<64, true>(ix, rd, bound, out, item_ctr, s)
-                          : launch_uniform<64, false>(ix, rd, bound, out, item_ctr, s);
+    case 16: return launch_uniform_fx<16>(ix, rd, bound, out, item_ctr, count, fx, s);
+    case 32: return launch_uniform_fx<32>(ix, rd, bound, out, item_ctr, count, fx, s);
+    default: return launch_uniform_fx<64>(ix, rd, bound, out, item_ctr, count, fx, s);
   }
 }
 
