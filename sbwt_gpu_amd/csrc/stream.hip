@@ -249,10 +249,22 @@ struct Walker {
     }
   }
 
-  // a search of the string of `length` characters at row position start;
-  // its window is loaded by load_window()
+  // the top-aligned window at offset o (< 64) of the held words w0, w1
+  // (characters past w1 read as A)
+  __device__ __forceinline__ u64 held_window(u32 o) const {
+    return o >= 32 ? w1 << (2 * (o - 32)) : (o ? (w0 << (2 * o)) | (w1 >> (64 - 2 * o)) : w0);
+  }
+
+  // a search of the string of `length` characters at row position start:
+  // its window comes from the held words when they hold the string (a
+  // certificate's string mostly lies within a few characters of j), else
+  // it is loaded by load_window()
   __device__ __forceinline__ void start_search(const DevIndex& ix, UTally& t, u32 pur, u32 start, u32 length) {
-    begin_search(ix, t, pur | kWinPending, length, start);
+    const u64 at = cbase + start, w = wi << 5;
+    if (wi != ~0ull - 1 && at >= w && at + length <= w + 64)
+      begin_search(ix, t, pur, length, held_window(static_cast<u32>(at - w)));
+    else
+      begin_search(ix, t, pur | kWinPending, length, start);
   }
 
   // the answer for the k-mer at j (through the lane's answer buffer); moves to j + 1
@@ -404,7 +416,7 @@ struct Walker {
           mode = M_STREAM;
         } else {
           // the window of the k-mer at j, from the two words
-          begin_search(ix, t, P_KMER, k, o ? (w0 << (2 * o)) | (w1 >> (64 - 2 * o)) : w0);
+          begin_search(ix, t, P_KMER, k, held_window(o));
         }
       }
     }
