@@ -431,46 +431,80 @@ def large_index(kind: str, args) -> dict:
     torch.cuda.synchronize()
     gen_s = time.time() - t
     log(f"{kind}: {G + R} bases generated on the GPU in {gen_s:.1f}s")
+    # The queries come first, before the index build's allocations: in a
+    # process that started right after another large one, a run of a few
+    # thousand entries of a query array written after the build was seen
+    # holding foreign data (the index itself was bit-identical to a good
+    # run's). Two independent extractions of the sampled queries must agree;
+    # a disagreement regenerates them.
+    half = (Q + 1) // 2
+    if kind == "uniform":
+        qpos = torch.empty(Q, dtype=torch.int64, device=dev)
+        qpos[0::2] = torch.randint(0, G - k + 1, (half,), device=dev, generator=gen)
+        qpos[1::2] = G + torch.randint(0, R - k + 1, (Q // 2,), device=dev, generator=gen)
+        spos = qpos[0::2].clone()  # the sampled queries' positions
+    else:
+        lens = torch.diff(starts)
+        hap = torch.randint(0, lens.numel(), (half,), device=dev, generator=gen)
+        off = (torch.rand(half, device=dev, generator=gen, dtype=torch.float64) * (lens - k + 1)[hap]).to(torch.int64)
+        spos = starts[:-1][hap] + off
+        del hap, off
+        qpos = None
+        rand_half = random_kmers(Q // 2, k, gen)
+    integrity = {"query_regenerations": 0}
+    for attempt in range(3):
+        packed = torch.empty(L.sbwt_packed_words(G + R), dtype=torch.int64, device=dev)
+        check(L.sbwt_pack(_ptr(chars), G + R, _ptr(packed), None, s))
+        kmers = torch.empty(Q, dtype=torch.int64, device=dev)
+        if qpos is not None:
+            check(L.sbwt_extract_kmers(_ptr(packed), _ptr(qpos), Q, _ptr(kmers), s))
+        else:
+            kpos = torch.empty(half, dtype=torch.int64, device=dev)
+            check(L.sbwt_extract_kmers(_ptr(packed), _ptr(spos), half, _ptr(kpos), s))
+            kmers[0::2] = kpos
+            kmers[1::2] = rand_half
+            del kpos
+        # the sampled queries once more, from a second packing of the characters
+        packed2 = torch.empty(L.sbwt_packed_words(G + R), dtype=torch.int64, device=dev)
+        check(L.sbwt_pack(_ptr(chars), G + R, _ptr(packed2), None, s))
+        again = torch.empty_like(spos)
+        check(L.sbwt_extract_kmers(_ptr(packed2), _ptr(spos), spos.numel(), _ptr(again), s))
+        torch.cuda.synchronize()
+        same = bool(torch.equal(again, kmers[0::2]))
+        del packed, packed2, again
+        if same:
+            break
+        integrity["query_regenerations"] += 1
+        log(f"{kind}: the two extractions of the sampled queries differ; generating them again")
+        del kmers
+    integrity["queries_equal_second_extraction"] = same
+    integrity["query_digest_before_search"] = int(kmers.sum())
+    del qpos
+    if kind != "uniform":
+        del rand_half
+    out = torch.empty_like(kmers)
+    maybe = torch.empty(Q, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
     t = time.time()
     # the default per index: the fastest of the layout sweep over several runs
     # (the others stay within 2-8%; the autotuner's pick varies among those
     # within 2% from run to run)
     default = {"uniform": (32, True), "pangenome": (64, False)}[kind]
     build_layout = {"default": default[0], "auto": 32}.get(args.large_layout) or int(args.large_layout)
+    in_sum0 = _byte_sum(chars[:G])
     ix = sg.Index.build(chars[:G], k, sg.Config(presearch_depth=args.presearch, layout=build_layout),
                         starts=starts)
+    # integrity: the build leaves its input and the queries alone, and the
+    # searches leave the index alone
+    integrity["input_unchanged_by_build"] = _byte_sum(chars[:G]) == in_sum0
+    integrity["queries_unchanged_by_build"] = int(kmers.sum()) == integrity["query_digest_before_search"]
+    integrity["index_digest_after_build"] = ix.digest()
     info = ix.info
     build_s = time.time() - t
     log(f"{kind}: index built on GPU in {build_s:.1f}s: n={info['n']} columns, "
         f"{info['device_bytes'] / 2**30:.2f} GiB, layout L{info['layout']}, m={info['presearch_depth']}")
-    packed = torch.empty(L.sbwt_packed_words(G + R), dtype=torch.int64, device=dev)
-    check(L.sbwt_pack(_ptr(chars), G + R, _ptr(packed), None, s))
-    del chars
-    half = (Q + 1) // 2
-    if kind == "uniform":
-        pos = torch.empty(Q, dtype=torch.int64, device=dev)
-        pos[0::2] = torch.randint(0, G - k + 1, (half,), device=dev, generator=gen)
-        pos[1::2] = G + torch.randint(0, R - k + 1, (Q // 2,), device=dev, generator=gen)
-        kmers = torch.empty_like(pos)
-        check(L.sbwt_extract_kmers(_ptr(packed), _ptr(pos), Q, _ptr(kmers), s))
-        del pos
-    else:
-        lens = torch.diff(starts)
-        hap = torch.randint(0, lens.numel(), (half,), device=dev, generator=gen)
-        off = (torch.rand(half, device=dev, generator=gen, dtype=torch.float64) * (lens - k + 1)[hap]).to(torch.int64)
-        pos = starts[:-1][hap] + off
-        del hap, off
-        kpos = torch.empty(half, dtype=torch.int64, device=dev)
-        check(L.sbwt_extract_kmers(_ptr(packed), _ptr(pos), half, _ptr(kpos), s))
-        kmers = torch.empty(Q, dtype=torch.int64, device=dev)
-        kmers[0::2] = kpos
-        kmers[1::2] = random_kmers(Q // 2, k, gen)
-        del pos, kpos
-    del packed, starts
-    out = torch.empty_like(kmers)
-    maybe = torch.empty(Q, dtype=torch.uint8, device=dev)
-    torch.cuda.synchronize()
-    torch.cuda.empty_cache()
+    del starts
     log(f"{kind}: {Q} queries ready")
     tune = None
     if args.large_layout == "auto":
@@ -503,6 +537,12 @@ def large_index(kind: str, args) -> dict:
     found = int((ref != -1).sum().item())
     log(f"{kind} plain: {Q / el / 1e9:.3f} G k-mers/s ({el * 1e3:.2f} ms/step, kernel {kern_ms:.2f} ms), "
         f"found {found}/{Q}, sampled k-mers missing: {missing}")
+    integrity["index_unchanged_by_queries"] = ix.digest() == integrity["index_digest_after_build"]
+    integrity["queries_unchanged_by_search"] = int(kmers.sum()) == integrity["query_digest_before_search"]
+    if missing:
+        integrity["missing"] = _recheck_missing(ix, chars, spos, kmers, ref, k, kflags, args.presearch, s)
+    log(f"{kind} integrity: {integrity}")
+    del chars, spos
     # algorithmic bytes: the counted variant on the same queries (untimed)
     ix.counters(reset=True)
     check(L.sbwt_search_kmers_flags(ix._h, _ptr(kmers), Q, _ptr(out), kflags | _capi.COUNT, s))
@@ -523,7 +563,7 @@ def large_index(kind: str, args) -> dict:
            "presearch": info["presearch_depth"],
            "kmer_autotune": tune,
            "queries": Q, "found_fraction": round(found / Q, 4), "sampled_kmers_all_found": missing == 0,
-           "roofline": roofline}
+           "integrity": integrity, "roofline": roofline}
 
     paths = {"plain": {"kmers_per_s": round(Q / el, 1), "ms_per_step": round(el * 1e3, 3),
                        "filter_kernel_ms": 0.0, "search_kernel_ms": round(kern_ms, 3), "maybe_fraction": None,
@@ -609,6 +649,64 @@ def large_index(kind: str, args) -> dict:
     ix.close()
     del kmers, out, ref, maybe
     torch.cuda.empty_cache()
+    return res
+
+
+def _byte_sum(t) -> int:
+    """Sum of the bytes of a device uint8 tensor (in pieces: no large temporaries)."""
+    import torch
+
+    step = 1 << 28
+    return sum(int(t[i:i + step].sum(dtype=torch.int64)) for i in range(0, t.numel(), step))
+
+
+def _recheck_missing(ix, chars, spos, kmers, ref, k, kflags, m, s, limit=4096):
+    """Where sampled queries were answered absent: are the queries the input's
+    k-mers (recomputed from the characters, not from the packed string), and
+    are they found by a second search, after the presearch table is rebuilt,
+    and after the rank chunks are rebuilt from the rows? Localises a wrong
+    answer to the queries, the table, the chunks or the rows."""
+    import torch
+
+    from sbwt_gpu_amd import _capi
+    from sbwt_gpu_amd.api import _ptr
+
+    L, check = _capi.load(), _capi.check
+    miss = torch.nonzero(ref[0::2] == -1).flatten()
+    n_all = int(miss.numel())
+    miss = miss[:limit]
+    pos = spos[miss]
+    # the k-mers from the characters: codes of pos .. pos+k-1, top-aligned
+    win = chars[pos[:, None] + torch.arange(k, device=pos.device)[None, :]].to(torch.int64)
+    code = (win == 67) * 1 + (win == 71) * 2 + (win == 84) * 3
+    shifts = 62 - 2 * torch.arange(k, device=pos.device)
+    again = (code << shifts[None, :]).sum(1)
+    mask = -(1 << (64 - 2 * k)) if k < 32 else -1
+    q = kmers[0::2][miss]
+    bad = q[(q & mask) != again]
+    res = {"count": n_all, "checked": int(miss.numel()),
+           "queries_equal_input": int(((q & mask) == again).sum()),
+           "first_positions": [int(x) for x in pos[:8].tolist()],
+           "first_sampled_indices": [int(x) for x in miss[:16].tolist()],
+           "first_wrong_values": [f"{int(x) & 0xFFFFFFFFFFFFFFFF:016x}" for x in bad[:8].tolist()],
+           "wrong_zero": int((bad == 0).sum()), "wrong_all_ones": int((bad == -1).sum()),
+           "wrong_below_2_32": int(((bad >= 0) & (bad < (1 << 32))).sum())}
+
+    def search_again():
+        out = torch.empty_like(q)
+        check(L.sbwt_search_kmers_flags(ix._h, _ptr(q), q.numel(), _ptr(out), kflags, s))
+        torch.cuda.synchronize()
+        return int((out != -1).sum())
+
+    res["found_on_second_search"] = search_again()
+    before = ix.digest()
+    ix.presearch(m)
+    after = ix.digest()
+    res["table_equal_after_rebuild"] = after["table"] == before["table"]
+    res["found_after_table_rebuild"] = search_again()
+    ix.set_layout(ix.info["layout"])
+    res["chunks_equal_after_rebuild"] = ix.digest()["chunks"] == before["chunks"]
+    res["found_after_chunk_rebuild"] = search_again()
     return res
 
 

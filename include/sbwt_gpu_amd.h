@@ -161,6 +161,16 @@ int sbwt_index_load(const char *path, const sbwt_config *cfg, sbwt_index **out);
 
 int sbwt_index_get_info(const sbwt_index *ix, sbwt_index_info *info);
 
+/* Digests of the index's device memory, for checking that nothing changed
+ * it between two points: out[0] = the rows, out[1] = the rank chunks,
+ * out[2] = the presearch table (0 without one). Each is the sum over the
+ * region's 64-bit words of splitmix64's output function applied to
+ * (word + index * 2^64/phi), so a changed word or an exchange of two words
+ * changes it. Synchronous (the null stream). No reference counterpart: the
+ * reference holds its index in 18 separate allocations it never re-reads
+ * for checking (include/gpu/device_memory.hpp:14-74). */
+int sbwt_index_digest(const sbwt_index *ix, uint64_t out[3]);
+
 /* Copies the rows back to host memory, nwords words each (>= ceil(n/64)),
  * MSB-first; group may be NULL. */
 int sbwt_index_get_rows(const sbwt_index *ix, uint64_t *const rows[4],

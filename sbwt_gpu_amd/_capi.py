@@ -50,6 +50,7 @@ SIGNATURES = {
     "sbwt_build_set_sort_chunk": (i32, [u64]),
     "sbwt_index_free": (i32, [vp]),
     "sbwt_index_get_info": (i32, [vp, C.POINTER(IndexInfo)]),
+    "sbwt_index_digest": (i32, [vp, u64p]),
     "sbwt_index_get_rows": (i32, [vp, u64p * 4, vp, u64]),
     "sbwt_index_write_files": (i32, [vp, cp * 4, cp]),
     "sbwt_presearch": (i32, [vp, u32, vp]),

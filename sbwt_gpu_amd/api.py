@@ -153,6 +153,14 @@ class Index:
         return {"n": i.n, "C": list(i.C), "device_bytes": i.device_bytes, "k": i.k,
                 "presearch_depth": i.presearch_depth, "layout": i.layout, "has_groups": bool(i.has_groups)}
 
+    def digest(self) -> dict:
+        """Digests of the rows, rank chunks and presearch table in device
+        memory (sbwt_index_digest): equal digests at two points mean
+        nothing changed the index in between."""
+        out = (C.c_uint64 * 3)()
+        check(self.lib.sbwt_index_digest(self._h, out))
+        return {"rows": int(out[0]), "chunks": int(out[1]), "table": int(out[2])}
+
     @property
     def n(self) -> int:
         return self.info["n"]

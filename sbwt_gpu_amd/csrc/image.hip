@@ -68,19 +68,21 @@ __global__ __launch_bounds__(256) void k_checksum(const u64* __restrict__ p, u64
 
 __global__ void k_zero(unsigned long long* p) { *p = 0; }
 
-int rows_checksum(const sbwt_index* ix, u64* out) {
+// The checksum of n words at p (synchronous, the null stream).
+static int words_checksum(const u64* p, u64 n, u64* out) {
   unsigned long long* d = nullptr;
   SBWT_HIP(hipMalloc(&d, sizeof *d));
-  const u64 n = kRows * ix->nwords;
   const u64 want = (n + 255) / 256;
   k_zero<<<1, 1>>>(d);
-  k_checksum<<<static_cast<u32>(want < 16384 ? want : 16384), 256>>>(ix->d_rows, n, d);
+  if (n) k_checksum<<<static_cast<u32>(want < 16384 ? want : 16384), 256>>>(p, n, d);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpy(out, d, sizeof *out, hipMemcpyDeviceToHost);
   (void)hipFree(d);
-  if (e != hipSuccess) return fail(SBWT_ERR_HIP, "the rows' checksum failed: %s", hipGetErrorString(e));
+  if (e != hipSuccess) return fail(SBWT_ERR_HIP, "a checksum of the index failed: %s", hipGetErrorString(e));
   return SBWT_OK;
 }
+
+int rows_checksum(const sbwt_index* ix, u64* out) { return words_checksum(ix->d_rows, kRows * ix->nwords, out); }
 
 // Two pinned staging buffers and the events of their copies.
 struct Staging {
@@ -250,6 +252,17 @@ int sbwt_index_load(const char* path, const sbwt_config* cfg, sbwt_index** out) 
     return rc;
   }
   *out = ix;
+  return SBWT_OK;
+}
+
+int sbwt_index_digest(const sbwt_index* ix, uint64_t out[3]) {
+  clear_error();
+  SBWT_REQUIRE(ix && out, "index and out must not be NULL");
+  SBWT_HIP(hipDeviceSynchronize());  // searches still running on other streams end first
+  SBWT_TRY(rows_checksum(ix, &out[0]));
+  SBWT_TRY(words_checksum(reinterpret_cast<const u64*>(ix->d_lines), kRows * ix->row_q * 2, &out[1]));
+  out[2] = 0;
+  if (ix->m && ix->d_pre) SBWT_TRY(words_checksum(reinterpret_cast<const u64*>(ix->d_pre), 1ull << (2 * ix->m), &out[2]));
   return SBWT_OK;
 }
 

@@ -183,8 +183,7 @@ __device__ __forceinline__ u32 part_rank(const uint4& v, u32 part, u32 o) {
       r += w[i];  // the count before the chunk
     } else {
       const u32 lo = 32 * (cw - 1);  // first column of payload word cw - 1
-      const u32 m = (o >= lo + 32) ? ~0u : ((o > lo) ? ((1u << (o - lo)) - 1u) : 0u);
-      r += __popc(w[i] & m);
+      r += __popc(o >= lo + 32 ? w[i] : (o > lo ? w[i] & ((1u << (o - lo)) - 1u) : 0u));
     }
   }
   return r;
@@ -228,7 +227,8 @@ __global__ __launch_bounds__(256) void k_search_kmers_coop(DevIndex ix, const u6
       }
       const uint4* row = row_of(ix, c);
       const uint4 va = row[static_cast<u64>(cha) * Layout<CB>::Q + part];
-      const uint4 vb = row[static_cast<u64>(chb) * Layout<CB>::Q + part];
+      uint4 vb = va;  // a single-column interval's l and r + 1 mostly share a chunk
+      if (chb != cha) vb = row[static_cast<u64>(chb) * Layout<CB>::Q + part];
       u32 ra = part_rank<CB>(va, part, a - cha * COLS), rb = part_rank<CB>(vb, part, b - chb * COLS);
 #pragma unroll
       for (u32 d = 1; d < G; d <<= 1) {

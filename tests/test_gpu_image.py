@@ -158,3 +158,38 @@ def test_image_round_trip_k(tmp_path, k):
     np.testing.assert_array_equal(sg.to_u64(back.search_reads(batch)), sg.to_u64(ix.search_reads(batch)))
     ix.close()
     back.close()
+
+
+
+def test_index_digest_tracks_every_region(small):
+    """sbwt_index_digest: unchanged by searches and by rebuilding a region
+    from the same rows; changed exactly where the layout, the table depth or
+    a row changes."""
+    s, g, q, k = small
+    m = 9
+    ix = sg.Index.from_rows(s.rows, s.n, k, group=s.group, config=sg.Config(presearch_depth=m, layout=16))
+    d0 = ix.digest()
+    assert d0["table"] != 0
+    ix.search_kmers(dev_u64(q))
+    torch.cuda.synchronize()
+    assert ix.digest() == d0
+    ix.presearch(m)
+    ix.set_layout(16)
+    assert ix.digest() == d0  # rebuilt from the same rows: the same words
+    ix.set_layout(32)
+    d1 = ix.digest()
+    assert d1["rows"] == d0["rows"] and d1["table"] == d0["table"] and d1["chunks"] != d0["chunks"]
+    ix.presearch(m - 1)
+    d2 = ix.digest()
+    assert d2["rows"] == d0["rows"] and d2["chunks"] == d1["chunks"] and d2["table"] != d0["table"]
+    ix.presearch(0)
+    assert ix.digest()["table"] == 0
+    # one bit moved inside a row (same C): another digest of the rows
+    rows = [r.copy() for r in s.rows]
+    w = next(i for i in range(rows[0].size // 2) if int(rows[0][i]) & ~(int(rows[0][i]) - 1) > 1)
+    x = int(rows[0][w])
+    lo = (x & -x).bit_length() - 1  # lowest set bit, > 0: bit lo - 1 is clear
+    rows[0][w] = np.uint64(x ^ (1 << lo) ^ (1 << (lo - 1)))
+    other = sg.Index.from_rows(rows, s.n, k, config=sg.Config(presearch_depth=0, layout=16))
+    base = sg.Index.from_rows(s.rows, s.n, k, config=sg.Config(presearch_depth=0, layout=16))
+    assert other.digest()["rows"] != base.digest()["rows"]

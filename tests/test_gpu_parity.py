@@ -795,3 +795,51 @@ def test_counted_lines_equal_the_oracle_count(layout, m):
     np.testing.assert_array_equal(got, want)
     assert ctr["lines"] == int(lines.sum()) and ctr["steps"] == int(steps.sum())
     assert ctr["kmers"] == q.size and ctr["full_searches"] == q.size
+
+
+def _short_rows(rng, g_codes, n, lo, hi, err):
+    """n rows of lo..hi characters from uniform positions of the genome (2-bit
+    codes), with substitutions at rate err, concatenated; returns the ASCII
+    characters and the row starts."""
+    lens = rng.integers(lo, hi + 1, n)
+    starts = rng.integers(0, g_codes.size - hi, n)
+    rs = np.zeros(n + 1, np.uint64)
+    rs[1:] = np.cumsum(lens)
+    total = int(rs[-1])
+    idx = np.repeat(starts - rs[:-1].astype(np.int64), lens) + np.arange(total)
+    codes = g_codes[idx].copy()
+    flip = rng.random(total) < err
+    codes[flip] = rng.integers(0, 4, int(flip.sum()))
+    return np.frombuffer(b"ACGT", np.uint8)[codes], rs
+
+
+@pytest.mark.parametrize("seg", [256, 7])
+def test_search_reads_many_short_rows(seg):
+    """More work items than the kernel has lanes, over short rows packed back
+    to back: lanes claim the item after the one they finished, whose first
+    k-mer often starts in the packed word after the last one they held (the
+    held words of the uniform kernel must be dropped on every change of
+    row). Every read-search form against the oracle, and run items through a
+    plan."""
+    rng = np.random.default_rng(71 + seg)
+    k = 31
+    g_codes = rng.integers(0, 4, 1_000_000).astype(np.uint8)
+    g = np.frombuffer(b"ACGT", np.uint8)[g_codes].tobytes().decode()
+    s = O.SBWT([g], k)
+    chars, rs = _short_rows(rng, g_codes, 450_000, k, 72, 0.01)
+    oix = oracle_index(s, 10)
+    want = O.search_reads(oix, chars, rs, k)
+    ix = sg.Index.from_rows(s.rows, s.n, k, group=s.group, config=sg.Config(presearch_depth=10))
+    dchars = torch.from_numpy(chars.copy()).cuda()
+    batch = sg.ReadBatch(dchars, dev_u64(rs), k, seg=seg, host_starts=rs)
+    assert batch.work_bound > 400_000  # more items than 256 CUs x 6 blocks x 256 lanes
+    for kw in ({}, {"persistent": True}, {"certificates": False, "persistent": True}, {"lanes": True}):
+        got = sg.to_u64(ix.search_reads(batch, **kw))
+        np.testing.assert_array_equal(got, want, err_msg=str(kw))
+    plan = sg.ReadsPlan(rs.size - 1, chars.size, seg)
+    out = torch.empty(want.size, dtype=torch.int64, device="cuda")
+    for kw in ({}, {"runs": True, "persistent": True}):
+        out.fill_(7)
+        plan.search(ix, dchars, dev_u64(rs), out, **kw)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(sg.to_u64(out), want, err_msg=f"plan {kw}")
