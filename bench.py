@@ -1057,7 +1057,8 @@ def run_batched(args) -> int:
         t = time.time()
         # depths up to 13: deeper tables win the trials (repeated searches of
         # one batch, whose table entries stay cached) but lose over the pass
-        tune = autotune.autotune(ix, sample_chars, sample_rel, segs=(64, 256, 1024), budget_s=90.0)
+        tune = autotune.autotune(ix, sample_chars, sample_rel, segs=(64, 256, 1024), budget_s=90.0,
+                                 streams=max(1, args.streams))
         seg = tune["best"]["segment_kmers"]
         kflags = autotune.kernel_flags(tune["best"]["kernel"])
         log(f"autotune ({time.time() - t:.1f}s, {len(tune['trials'])} trials): {tune['best']}")

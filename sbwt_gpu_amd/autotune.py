@@ -32,14 +32,26 @@ def kernel_flags(kernel: str) -> dict:
 
 def autotune(index: Index, chars: torch.Tensor, row_starts: torch.Tensor, *, layouts=(16, 32, 64),
              depths=(10, 11, 12, 13), segs=(64, 256, 1024), kernels=KERNELS, reps: int = 3,
-             max_kmers: int | None = None, budget_s: float = 60.0, n_as_a: bool = False, log=None) -> dict:
+             max_kmers: int | None = None, budget_s: float = 60.0, n_as_a: bool = False, streams: int = 1,
+             log=None) -> dict:
     """chars / row_starts: a sample batch on the device (row_starts int64,
     offsets into chars). Returns {"best": {...}, "trials": [...]} and sets
-    the index to the best layout and presearch depth."""
+    the index to the best layout and presearch depth.
+
+    streams > 1 times every trial the way batches run on several plans and
+    streams: reps searches of the sample on each of `streams` streams at
+    once, whose launches overlap each other's ends; "ms" is then the time
+    per search. With one stream a trial's time includes the launch's end,
+    where the last lanes finish alone, and the ranking can differ (a
+    batched pass on three streams ran 4% slower with the one-stream pick)."""
     rows = row_starts.numel() - 1
     n_chars = chars.numel()
     cap = max_kmers if max_kmers is not None else n_chars
-    out = torch.empty(max(cap, 1), dtype=torch.int64, device=chars.device)
+    streams = max(1, streams)
+    outs = [torch.empty(max(cap, 1), dtype=torch.int64, device=chars.device) for _ in range(streams)]
+    out = outs[0]
+    lanes = [torch.cuda.Stream() for _ in range(streams)] if streams > 1 else []
+    total = int(torch.clamp(torch.diff(row_starts) - index.k + 1, min=0).sum())  # answers per search
     ref = None
     trials = []
     t_start = time.time()
@@ -54,26 +66,42 @@ def autotune(index: Index, chars: torch.Tensor, row_starts: torch.Tensor, *, lay
         if index.info["presearch_depth"] != m:
             index.presearch(m)
         for seg, kern in itertools.product(segs, kernels):
-            plan = ReadsPlan(rows, n_chars, seg)
+            plans = [ReadsPlan(rows, n_chars, seg) for _ in range(streams)]
             flags = {**kernel_flags(kern), "n_as_a": n_as_a}
-            plan.search(index, chars, row_starts, out, **flags)  # warm
-            times = []
-            for _ in range(reps):
+            plans[0].search(index, chars, row_starts, out, **flags)  # warm
+            if streams == 1:
+                times = []
+                for _ in range(reps):
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record(stream)
+                    plans[0].search(index, chars, row_starts, out, **flags)
+                    e1.record(stream)
+                    e1.synchronize()
+                    times.append(e0.elapsed_time(e1))
+                ms = sorted(times)[len(times) // 2]
+            else:
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 e0.record(stream)
-                plan.search(index, chars, row_starts, out, **flags)
+                for st in lanes:
+                    st.wait_stream(stream)
+                for _ in range(reps):
+                    for p, o, st in zip(plans, outs, lanes):
+                        p.search(index, chars, row_starts, o, stream=st, **flags)
+                for st in lanes:
+                    stream.wait_stream(st)
                 e1.record(stream)
                 e1.synchronize()
-                times.append(e0.elapsed_time(e1))
-            if ref is None:
-                ref = out.clone()
-            elif not torch.equal(out, ref):
-                raise RuntimeError(f"autotune: answers differ for layout={layout} m={m} seg={seg} kernel={kern}")
-            ms = sorted(times)[len(times) // 2]
+                ms = e0.elapsed_time(e1) / (reps * streams)
+            for o in outs if streams > 1 else [out]:
+                if ref is None:
+                    ref = o[:total].clone()
+                elif not torch.equal(o[:total], ref):
+                    raise RuntimeError(f"autotune: answers differ for layout={layout} m={m} seg={seg} kernel={kern}")
             trials.append({"layout": layout, "presearch_depth": m, "segment_kmers": seg, "kernel": kern, "ms": ms})
             if log:
                 log(f"autotune layout=L{layout} m={m} seg={seg} {kern}: {ms:.3f} ms")
-            plan.close()
+            for p in plans:
+                p.close()
     best = min(trials, key=lambda t: t["ms"])
     if index.info["layout"] != best["layout"]:
         index.set_layout(best["layout"])

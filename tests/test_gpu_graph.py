@@ -127,9 +127,14 @@ def test_plan_rejects_oversized_batch():
         plan.search(ix, chars, starts, out)
 
 
-def test_autotune_keeps_answers_and_applies_best():
+@pytest.mark.parametrize("streams", [1, 3])
+def test_autotune_keeps_answers_and_applies_best(streams):
+    """The trials (one stream, or three concurrent streams with a plan and an
+    output each) all give the same answers, the best is applied, and it
+    answers as the oracle does."""
     rng, s, ix, oix, chars, starts, set_starts = _setup(seed=3)
-    res = autotune.autotune(ix, chars, starts, layouts=(16, 32), depths=(8, 10), segs=(64, 256), reps=1)
+    res = autotune.autotune(ix, chars, starts, layouts=(16, 32), depths=(8, 10), segs=(64, 256), reps=1,
+                            streams=streams)
     best = res["best"]
     assert len(res["trials"]) == 2 * 2 * 2 * len(autotune.KERNELS)
     assert ix.info["layout"] == best["layout"] and ix.info["presearch_depth"] == best["presearch_depth"]
