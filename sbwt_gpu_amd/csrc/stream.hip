@@ -381,6 +381,24 @@ struct Walker {
       j = jend < jv ? jend : jv;
     }
     while (mode == M_IDLE && j < jend) {
+      // the packed words of the k-mer at j, loaded before the non-ACGT scan
+      // below waits for its mask words, so that a new item's two loads
+      // overlap
+      const u64 at = cbase + j;
+      const u64 wt = at >> 5;
+      if (wt != wi) {
+        if (wt == wi + 1) {
+          w0 = w1;
+          w1 = wn;
+        } else {
+          w0 = rd.packed[wt];
+          w1 = rd.packed[wt + 1];
+        }
+        // (only when the row reaches that word: the batch's packed array
+        // ends one word after its last character's word)
+        wn = ((wt + 2) << 5) <= cbase + rowk + k - 2 ? rd.packed[wt + 2] : 0ull;
+        wi = wt;
+      }
       // a run of k-mers answered ~0 without a search: certified absent,
       // holding a non-ACGT character (every k-mer from j up to that
       // character holds it), or filtered out
@@ -394,21 +412,6 @@ struct Walker {
         pnode = knode;
         answer_run(out, rd, ab, t, run ? (run < jend - j ? run : jend - j) : 1u, run ? kNone : knode);
       } else {
-        const u64 at = cbase + j;
-        const u64 wt = at >> 5;
-        if (wt != wi) {
-          if (wt == wi + 1) {
-            w0 = w1;
-            w1 = wn;
-          } else {
-            w0 = rd.packed[wt];
-            w1 = rd.packed[wt + 1];
-          }
-          // (only when the row reaches that word: the batch's packed array
-          // ends one word after its last character's word)
-          wn = ((wt + 2) << 5) <= cbase + rowk + k - 2 ? rd.packed[wt + 2] : 0ull;
-          wi = wt;
-        }
         const u32 o = static_cast<u32>(at & 31);  // j's offset in w0
         if (F::stream(rd) && prev) {
           const u32 q = o + k - 1;  // the new character, at most 62
