@@ -837,9 +837,12 @@ def test_search_reads_many_short_rows(seg):
         got = sg.to_u64(ix.search_reads(batch, **kw))
         np.testing.assert_array_equal(got, want, err_msg=str(kw))
     plan = sg.ReadsPlan(rs.size - 1, chars.size, seg)
-    out = torch.empty(want.size, dtype=torch.int64, device="cuda")
+    buf = torch.empty(want.size + 2, dtype=torch.int64, device="cuda")
     for kw in ({}, {"runs": True, "persistent": True}):
-        out.fill_(7)
-        plan.search(ix, dchars, dev_u64(rs), out, **kw)
-        torch.cuda.synchronize()
-        np.testing.assert_array_equal(sg.to_u64(out), want, err_msg=f"plan {kw}")
+        for at in (0, 1):  # answers 16-byte aligned, or 8 bytes past (a batch inside a larger array)
+            buf.fill_(7)
+            out = buf[at:at + want.size]
+            plan.search(ix, dchars, dev_u64(rs), out, **kw)
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(sg.to_u64(out), want, err_msg=f"plan {kw} at {at}")
+            assert int(buf[at + want.size]) == 7 and (at == 0 or int(buf[0]) == 7)
