@@ -64,6 +64,12 @@ struct ReadsDev {
   u32 seg;
   u32 nbreak, stream, cert;
   u32 vec_out;  // out is 16-byte aligned (full answer groups go out as two 16-byte stores)
+  // 1: out is the caller's answer array minus one slot, so that an array
+  // 8 bytes off a 16-byte boundary still takes 16-byte stores; answer q goes
+  // to out[q + 1]. A group holding slot 0 is never full (the lane never
+  // writes slot 0), so it goes out slot by slot and nothing before the
+  // caller's array is written.
+  u32 oshift;
   const uint4* runs;  // run items {row, first k-mer, k-mers}, or null: segments of the work prefix
   const unsigned long long* run_count;
   u64 run_cap;  // items the run buffer holds
@@ -203,14 +209,14 @@ struct Walker {
       jend = rowk - j < rd.seg ? rowk : j + rd.seg;
     }
     cbase = rd.row_starts[row];
-    obase = o0;
+    obase = o0 + rd.oshift;
     wi = ~0ull - 1;  // the words held belong to the last row (wn only if that row reached it)
     prev = false;
     clo = 1;
     chi = 0;
     kpos = kNoPos;
     nN = kNoPos;
-    if (F::maybe(rd)) mf.reset(F::maybe(rd), obase + j);
+    if (F::maybe(rd)) mf.reset(F::maybe(rd), obase - rd.oshift + j);
   }
 
   // k-mers [from, to) of the row of rk k-mers at characters cb, answers at
@@ -231,7 +237,7 @@ struct Walker {
     chi = 0;
     kpos = kNoPos;
     nN = kNoPos;
-    if (F::maybe(rd)) mf.reset(F::maybe(rd), obase + j);
+    if (F::maybe(rd)) mf.reset(F::maybe(rd), obase - rd.oshift + j);
   }
 
   // a search of `length` characters whose top-aligned window is `window`
@@ -405,7 +411,7 @@ struct Walker {
       u32 run = 0;
       if (j >= clo && j <= chi) run = chi + 1 - j;
       else if (F::nbreak(rd) && has_n(rd, k)) run = nN + 1 - j;
-      else if (F::maybe(rd)) run = mf.zeros(F::maybe(rd), nk, obase + j);
+      else if (F::maybe(rd)) run = mf.zeros(F::maybe(rd), nk, obase - rd.oshift + j);
       const bool known = j == kpos;  // found by a certificate search
       if (run || known) {
         prev = !run;
@@ -760,9 +766,12 @@ int search_reads_persistent(const sbwt_index* ix, const u64* packed, const u64* 
                             const u64* off, const u64* work, const u32* items, u64 rows, u64 bound, u32 seg,
                             const u8* maybe, u64* out, bool nbreak, bool stream, bool cert, bool count,
                             const RunList& runs, unsigned long long* item_ctr, hipStream_t s) {
+  const uintptr_t oa = reinterpret_cast<uintptr_t>(out) & 15;
+  const u32 oshift = oa == 8 ? 1u : 0u;
   const ReadsDev rd{packed, nmask, rs, off, work, items, maybe, rows, seg,
                     nbreak ? 1u : 0u, stream ? 1u : 0u, cert ? 1u : 0u,
-                    (reinterpret_cast<uintptr_t>(out) & 15) == 0 ? 1u : 0u, runs.runs, runs.count, runs.cap};
+                    oa == 0 || oa == 8 ? 1u : 0u, oshift, runs.runs, runs.count, runs.cap};
+  out = reinterpret_cast<u64*>(reinterpret_cast<uintptr_t>(out) - 8 * oshift);
   const bool fixed = stream && cert && !maybe && rd.vec_out;
   const int fx = !fixed ? FX_ANY : (runs.runs ? FX_RUNS : (nbreak && items ? FX_SEGMENTS : FX_ANY));
   switch (ix->layout) {
