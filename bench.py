@@ -896,6 +896,8 @@ def run_uniform(args) -> int:
 
 def run_pangenome(args) -> int:
     res = large_index("pangenome", args)
+    if args.pmc_child:
+        return 0
     line = _large_line(res, args)
     line["config"] = {k: v for k, v in res.items() if k not in ("roofline", "cpu_baseline")}
     line["roofline"], line["cpu_baseline"] = res["roofline"], res.get("cpu_baseline")
