@@ -180,7 +180,7 @@ __global__ void k_presearch(DevIndex ix, u32 m, u64 count, uint2* __restrict__ o
 
 static int run_presearch(sbwt_index* ix, u32 m, hipStream_t s) {
   // a table of the same depth is rewritten in place: freeing and
-  // reallocating tens of GB costs the driver seconds
+  // reallocating tens of GB costs the HIP runtime seconds
   if (ix->d_pre && ix->pre_alloc_m != m) {
     SBWT_HIP(hipFree(ix->d_pre));
     ix->d_pre = nullptr;
