@@ -389,7 +389,10 @@ def test_search_reads_mixed_negative_reads(negative):
 
 def test_search_reads_unaligned_output():
     """Answers written through an output pointer that is 8- but not
-    16-byte aligned (the per-slot store path) equal the aligned ones."""
+    16-byte aligned equal the aligned ones, and nothing outside the array is
+    written: the per-lane kernel stores such answers slot by slot, the
+    uniform-step kernel passes the array one slot lower and keeps its 16-byte
+    stores for whole groups."""
     rng = np.random.default_rng(31)
     g = genome(rng, 60_000)
     k = 31
